@@ -1,0 +1,235 @@
+"""Benchmark: D3Q19 TRT-LES on one MI355X, synthetic vehicle at 320x160x128.
+
+Workload (the large plain-method case): FP32 populations, 320x160x128 cells
+(6 553 600), u_in = 0.08, seeded superellipsoid vehicle with four wheels
+(200 000 triangles, voxelized on the GPU) on a no-slip floor (PAREDE),
+free-slip sides and top, Smagorinsky Cs = 0.1, Lambda = 3/16.  The requested
+Re ~ 1e5 needs omega = 1.9991; the solver clamps omega at 1.90 as the
+reference's Units does (units.js:157-164), so the run is at Re_lattice ~ 876.
+
+A "step" is one lattice update of the whole domain.  The timed region holds K
+steps in chunks of 100 (the smoothstep inflow ramp reconfigures between
+chunks, as the reference's passo() does) plus a momentum-exchange force sample
+after every chunk of the last 5000 steps.  value = MLUPS = cells * K / wall.
+
+Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: this
+build runs on a single GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0          # MI355X published HBM3E bandwidth
+BYTES_PER_CELL = {"fp32": 152, "fp16": 76}   # 19 pops read + 19 written
+GRID = (320, 160, 128)
+U_IN = 0.08
+RE_REQUESTED = 1.0e5
+BODY_LENGTH = 96.0
+CHUNK = 100
+
+
+def parse_args():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--precision", choices=("fp32", "fp16"), default="fp32")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="budget of the CPU-baseline sample (0 disables it)")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(precision: str):
+    """HBM bytes per step-kernel launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), read
+    from the committed summary; None if it is absent."""
+    f = ROOT / "profiles" / f"step_{precision}_320x160x128_pmc.md"
+    if not f.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read_text())
+    return float(m.group(1)) if m else None
+
+
+def build_case(precision: str):
+    from cfd__amd import meshes, tunel
+    from cfd__amd.units import Units
+    from cfd__amd.voxel import voxelizar
+
+    nx, ny, nz = GRID
+    g = {"nx": nx, "ny": ny, "nz": nz}
+    pos, idx, info = meshes.vehicle(seed=7)
+    vox = voxelizar(pos, idx, g, banda=1.5, selarFrestas=True, comSdf=False)
+    tipo, esponja, cont = tunel.montarTipos(g, vox["tipo"], esteira=False)
+    area = tunel.areaFrontalCorpo(vox["tipo"], g)
+    # Physical scale chosen so that Re_physical = 1e5 for the 96-cell body.
+    length_m = 4.5
+    speed = RE_REQUESTED * 1.506e-5 / length_m
+    units = Units(lengthM=length_m, speedMs=speed, cellsPerLength=BODY_LENGTH, uLb=U_IN,
+                  rollingRoad=False, lesCs=0.1)
+    return dict(grid=g, tipo=tipo, esponja=esponja, area=area, units=units, vox=vox,
+                mesh=info, cont=cont)
+
+
+def run_gpu(case, precision: str, steps: int, warmup: int):
+    from cfd__amd.forces import JanelaCd, coeficientes
+    from cfd__amd.solver import SolverMI355X
+
+    g, units = case["grid"], case["units"]
+    s = SolverMI355X.criar(g, precisao=precision)
+    cfg = dict(omegaPlus=units.omegaPlus, lesCs=0.1, esponja=case["esponja"],
+               beltU=[0.0, 0.0, 0.0])
+
+    def aplicar(f):
+        s.configurar(inletU=[U_IN * f, 0.0, 0.0], **cfg)
+
+    s.definirTipos(case["tipo"])
+    aplicar(1.0)
+    s.calibrarRepouso()
+    s.rampa(int(math.floor(g["nx"] / 0.05 + 0.5)), aplicar)
+    s.inicializar()
+
+    # warm-up (untimed)
+    done = 0
+    while done < warmup:
+        n = min(CHUNK, warmup - done)
+        s.passo(n)
+        done += n
+    s.sincronizar()
+
+    cd_win, cl_win = JanelaCd(10 ** 6), JanelaCd(10 ** 6)
+    sample_from = max(0, steps - 5000)
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        n = min(CHUNK, steps - done)
+        kernel_ms += s.passoCronometrado(n)
+        done += n
+        if done > sample_from:
+            f = s.medirForcas()
+            co = coeficientes(f, units, case["area"], g, corrigirBloqueio=True, theta=2.5)
+            cd_win.adicionar(co["cd"])
+            cl_win.adicionar(co["cl"])
+    s.sincronizar()
+    wall = time.perf_counter() - t0
+    mass, nonfinite = s.massa()
+    s.destruir()
+    return dict(wall=wall, kernel_ms=kernel_ms, cd=cd_win.media, cd_std=cd_win.desvio,
+                cl=cl_win.media, cl_std=cl_win.desvio, n_samples=cd_win.n, mass=mass,
+                nonfinite=nonfinite)
+
+
+def cpu_baseline(case, budget_s: float):
+    """The oracle's float32 step (OpenMP, all host cores) on the same cell
+    types, from the initialized state, timed for about budget_s seconds."""
+    import oracle
+
+    nx, ny, nz = GRID
+    par = oracle.params_array(omegaPlus=case["units"].omegaPlus, lesCs=0.1,
+                              inletU=[U_IN, 0.0, 0.0], esponja=case["esponja"], nx=nx)
+    a = oracle.init(GRID, case["tipo"], par)
+    b = a.copy()
+    lib = oracle.lib()
+    t = np.ascontiguousarray(case["tipo"], dtype=np.uint32)
+    P = oracle._p
+    lib.orc_step_f32(nx, ny, nz, P(a), P(b), P(t), P(par), None)  # warm-up
+    a, b = b, a
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        lib.orc_step_f32(nx, ny, nz, P(a), P(b), P(t), P(par), None)
+        a, b = b, a
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 1000:
+            break
+    cells = nx * ny * nz
+    return {"value": cells * n / el / 1e6, "unit": "MLUPS", "cores": oracle.threads(),
+            "kind": "port",
+            "sample": f"oracle float32 step (oracle/lbm_oracle.c, OpenMP) on the same "
+                      f"320x160x128 vehicle cell types, {n} steps after 1 warm-up step, "
+                      f"{el:.1f} s"}
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note, not built).", file=sys.stderr)
+        sys.exit(2)
+    from cfd__amd import _lib
+
+    precision = args.precision
+    case = build_case(precision)
+    nx, ny, nz = GRID
+    cells = nx * ny * nz
+    r = run_gpu(case, precision, args.steps, args.warmup)
+    mlups = cells * args.steps / r["wall"] / 1e6
+    kernel_avg_s = r["kernel_ms"] / 1e3 / args.steps
+    bpc = BYTES_PER_CELL[precision]
+    achieved = bpc * cells / kernel_avg_s / 1e9
+    copy_gbs, _ = _lib.copy_bandwidth(1 << 30, 20)
+    cpu = cpu_baseline(case, args.cpu_seconds) if args.cpu_seconds > 0 else None
+    u = case["units"]
+    line = {
+        "metric": "MLUPS (D3Q19 TRT-LES, 6.55M cells)",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["wall"] * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if precision == "fp32" else "f16-storage/f32-collide",
+        "data": "synthetic (seeded 200k-triangle vehicle, GPU-voxelized)",
+        "config": {
+            "workload": f"plain D3Q19 TRT(3/16)+Smagorinsky(0.1) {precision} vehicle "
+                        f"{nx}x{ny}x{nz}, u_in={U_IN}",
+            "grid": [nx, ny, nz], "cells": cells, "precision": precision,
+            "omega_plus_used": u.omegaPlus, "re_lattice_used": u.reLattice,
+            "omega_plus_requested": 1 / (3 * U_IN * BODY_LENGTH / RE_REQUESTED + 0.5),
+            "re_requested": RE_REQUESTED, "floor": "PAREDE (no-slip)",
+            "parallelism": "single GPU",
+        },
+        "hbm_frac_of_peak": round(achieved / HBM_PEAK_GBS, 4),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(precision),
+            "kernel": "k_step (fused pull-stream + TRT + LES + boundaries)",
+            "bytes_per_cell": bpc,
+            "avg_kernel_us": round(kernel_avg_s * 1e6, 2),
+        },
+        "cpu_baseline": cpu,
+        "results": {
+            "cd_mean_last_5000": r["cd"], "cd_std": r["cd_std"],
+            "cl_mean_last_5000": r["cl"], "cl_std": r["cl_std"], "force_samples": r["n_samples"],
+            "frontal_area_cells": case["area"], "solid_cells": case["cont"]["solido"],
+            "voxelize_ms": case["vox"]["stats"]["msTotal"],
+            "mass_final": r["mass"], "nonfinite_cells": r["nonfinite"],
+            "copy_kernel_gbs": round(copy_gbs, 1),
+            "frac_of_copy": round(achieved / copy_gbs, 4),
+        },
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

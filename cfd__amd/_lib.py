@@ -1,0 +1,114 @@
+"""ctypes binding of ``libcfdmi.so`` (the C ABI of ``include/cfd_mi355x.h``).
+
+The library is required: importing this module raises if it has not been
+built (``python -m cfd__amd.build``).  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libcfdmi.so"
+
+CFD_STORE_FP32 = 0
+CFD_STORE_FP16 = 1
+
+
+class CFDError(RuntimeError):
+    """A failure reported by the native library (rc < 0)."""
+
+    def __init__(self, rc: int, msg: str):
+        super().__init__(msg or f"cfd error {rc}")
+        self.rc = rc
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("omega_plus", C.c_double), ("magic", C.c_double), ("les_cs", C.c_double),
+        ("belt", C.c_double * 3), ("inlet", C.c_double * 3),
+        ("sponge_start", C.c_double), ("sponge_len", C.c_double),
+        ("bl_thickness", C.c_double),
+    ]
+
+
+class VoxStats(C.Structure):
+    _fields_ = [
+        ("n_cells", C.c_int64), ("n_casca", C.c_int64), ("n_dentro", C.c_int64),
+        ("n_solido", C.c_int64), ("triangles_outside", C.c_int64),
+        ("hollow_suspect", C.c_int32), ("flood_iterations", C.c_int32),
+        ("ms_raster", C.c_double), ("ms_flood", C.c_double), ("ms_total", C.c_double),
+    ]
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m cfd__amd.build` (hipcc, gfx950)")
+    lib = C.CDLL(str(LIB_PATH))
+    P = C.c_void_p
+    sig = {
+        "cfd_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint32, C.POINTER(P)]),
+        "cfd_configure": (C.c_int, [P, C.POINTER(Params)]),
+        "cfd_set_types": (C.c_int, [P, P, C.c_size_t]),
+        "cfd_init_equilibrium": (C.c_int, [P]),
+        "cfd_set_populations": (C.c_int, [P, P, C.c_size_t]),
+        "cfd_get_populations": (C.c_int, [P, P, C.c_size_t]),
+        "cfd_set_front": (C.c_int, [P, C.c_int]),
+        "cfd_get_front": (C.c_int, [P, C.POINTER(C.c_int)]),
+        "cfd_step": (C.c_int, [P, C.c_int]),
+        "cfd_step_timed": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
+        "cfd_forces": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "cfd_update_macros": (C.c_int, [P]),
+        "cfd_read_macros": (C.c_int, [P, P, C.c_size_t]),
+        "cfd_mass": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+        "cfd_synchronize": (C.c_int, [P]),
+        "cfd_bytes": (C.c_int, [P, C.POINTER(C.c_int64)]),
+        "cfd_destroy": (None, [P]),
+        "cfd_voxelize": (C.c_int, [P, C.c_size_t, P, C.c_size_t, C.c_int, C.c_int, C.c_int,
+                                   C.c_double, C.c_int, P, P, C.POINTER(VoxStats)]),
+        "cfd_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]),
+        "cfd_last_error": (C.c_char_p, []),
+        "cfd_version": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+# Every entry point the header declares, for the export check in the tests.
+EXPORTS = (
+    "cfd_create", "cfd_configure", "cfd_set_types", "cfd_init_equilibrium",
+    "cfd_set_populations", "cfd_get_populations", "cfd_set_front", "cfd_get_front",
+    "cfd_step", "cfd_step_timed", "cfd_forces", "cfd_update_macros", "cfd_read_macros",
+    "cfd_mass", "cfd_synchronize", "cfd_bytes", "cfd_destroy", "cfd_voxelize",
+    "cfd_copy_bandwidth", "cfd_last_error", "cfd_version",
+)
+
+
+def check(rc: int) -> None:
+    if rc < 0:
+        raise CFDError(rc, lib.cfd_last_error().decode(errors="replace"))
+
+
+def ptr(a: np.ndarray) -> int:
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data
+
+
+def version() -> str:
+    return lib.cfd_version().decode()
+
+
+def copy_bandwidth(nbytes: int, iters: int = 20) -> tuple[float, float]:
+    """(GB/s, ms per copy) of a plain device copy of ``nbytes`` bytes."""
+    g, ms = C.c_double(), C.c_double()
+    check(lib.cfd_copy_bandwidth(nbytes, iters, C.byref(g), C.byref(ms)))
+    return g.value, ms.value

@@ -1,0 +1,442 @@
+// Solver runtime behind the C ABI of include/cfd_mi355x.h.
+//
+// Mirrors the resource model of SolverWebGPU (src/backend/webgpu/solver.js):
+// two population sets with a front flag, a type field, a macro field and a
+// reduction scratch area, all owned by the handle.  Each handle has its own
+// HIP stream (default priority); everything it does is ordered on it.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/cfd_mi355x.h"
+#include "kernels.h"
+#include "lattice.h"
+
+using namespace cfd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCK(expr)                                                                  \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(e_ == hipErrorOutOfMemory ? CFD_E_OOM : CFD_E_HIP, "%s: %s (%s)",   \
+                  __func__, hipGetErrorString(e_), #expr);                           \
+  } while (0)
+
+}  // namespace
+
+struct cfd_solver {
+  int nx = 0, ny = 0, nz = 0;
+  size_t N = 0;
+  Store store = Store::F32;
+  size_t pop_bytes = 0;  // bytes of one population set
+  Geom geom{};
+  StepParams params{};
+  bool configured = false;
+  int front = 0;  // 0 = A, 1 = B
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void* pop[2] = {nullptr, nullptr};
+  uint8_t* tipo8 = nullptr;
+  float4* macros = nullptr;
+  double* partials = nullptr;
+  double* result = nullptr;   // device, 4 doubles
+  double* host_res = nullptr; // pinned, 4 doubles
+};
+
+namespace {
+
+void release(cfd_solver* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (void* p : s->pop)
+    if (p) (void)hipFree(p);
+  if (s->tipo8) (void)hipFree(s->tipo8);
+  if (s->macros) (void)hipFree(s->macros);
+  if (s->partials) (void)hipFree(s->partials);
+  if (s->result) (void)hipFree(s->result);
+  if (s->host_res) (void)hipHostFree(s->host_res);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int64_t device_bytes(const cfd_solver* s) {
+  return (int64_t)(2 * s->pop_bytes + s->N * (1 + 16) +
+                   (kReduceGrid * 3 + 4) * sizeof(double));
+}
+
+// c_i . v in the emitted order (ir.js:116-126), float32, host side.
+float dotc_host(int i, const float v[3]) {
+  float acc = 0.0f;
+  bool first = true;
+  const int c[3] = {cx(i), cy(i), cz(i)};
+  for (int a = 0; a < 3; ++a) {
+    if (c[a] == 0) continue;
+    if (first) {
+      acc = c[a] > 0 ? v[a] : -v[a];
+      first = false;
+    } else {
+      acc = c[a] > 0 ? acc + v[a] : acc - v[a];
+    }
+  }
+  return acc;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cfd_last_error(void) { return g_err.c_str(); }
+
+// Lets the voxelizer translation unit report through cfd_last_error().
+void cfd__set_error(const char* msg) { g_err = msg ? msg : ""; }
+
+const char* cfd_version(void) { return "cfd_mi355x 0.1 gfx950"; }
+
+int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
+  g_err.clear();
+  if (!out) return fail(CFD_E_ARG, "cfd_create: out is null");
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || nz < 1)
+    return fail(CFD_E_ARG, "cfd_create: grid %dx%dx%d must be positive", nx, ny, nz);
+  const uint64_t N = (uint64_t)nx * (uint64_t)ny * (uint64_t)nz;
+  // Byte offsets of one population slice are 32-bit in the kernels.
+  if (N >= (1ull << 30))
+    return fail(CFD_E_ARG, "cfd_create: %llu cells exceed the 2^30 limit of one handle",
+                (unsigned long long)N);
+  if (flags != CFD_STORE_FP32 && flags != CFD_STORE_FP16)
+    return fail(CFD_E_ARG, "cfd_create: unknown storage flags 0x%x", flags);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(CFD_E_HIP, "cfd_create: no HIP device visible");
+
+  cfd_solver* s = new cfd_solver();
+  s->nx = nx;
+  s->ny = ny;
+  s->nz = nz;
+  s->N = (size_t)N;
+  s->store = flags == CFD_STORE_FP16 ? Store::F16 : Store::F32;
+  s->pop_bytes = (size_t)Q * s->N * (s->store == Store::F16 ? 2 : 4);
+  s->geom = make_geom(nx, ny, nz);
+
+  auto bail = [&](int code) {
+    release(s);
+    return code;
+  };
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail(fail(CFD_E_HIP, "cfd_create: %s", hipGetErrorString(e)));
+  if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
+    return bail(fail(CFD_E_HIP, "cfd_create: event creation failed"));
+
+  struct {
+    void** p;
+    size_t bytes;
+  } allocs[] = {
+      {&s->pop[0], s->pop_bytes},
+      {&s->pop[1], s->pop_bytes},
+      {(void**)&s->tipo8, s->N},
+      {(void**)&s->macros, s->N * sizeof(float4)},
+      {(void**)&s->partials, (size_t)kReduceGrid * 3 * sizeof(double)},
+      {(void**)&s->result, 4 * sizeof(double)},
+  };
+  for (auto& a : allocs) {
+    e = hipMalloc(a.p, a.bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return bail(fail(CFD_E_OOM,
+                       "device memory insufficient for %.2f GB (%dx%dx%d); choose a "
+                       "smaller grid",
+                       device_bytes(s) / 1e9, nx, ny, nz));
+    }
+    e = hipMemsetAsync(*a.p, 0, a.bytes, s->stream);
+    if (e != hipSuccess) return bail(fail(CFD_E_HIP, "cfd_create: memset failed"));
+  }
+  if (hipHostMalloc((void**)&s->host_res, 4 * sizeof(double), hipHostMallocDefault) !=
+      hipSuccess)
+    return bail(fail(CFD_E_HIP, "cfd_create: pinned allocation failed"));
+  e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return bail(fail(CFD_E_HIP, "cfd_create: %s", hipGetErrorString(e)));
+  *out = s;
+  return CFD_OK;
+}
+
+int cfd_configure(cfd_solver* s, const cfd_params* p) {
+  g_err.clear();
+  if (!s || !p) return fail(CFD_E_ARG, "cfd_configure: null argument");
+  StepParams q{};
+  q.omega_plus = (float)p->omega_plus;
+  q.magic = (float)p->magic;
+  q.les_cs = (float)p->les_cs;
+  for (int a = 0; a < 3; ++a) {
+    q.belt[a] = (float)p->belt[a];
+    q.inlet[a] = (float)p->inlet[a];
+  }
+  q.sponge_start = (float)p->sponge_start;
+  const float len = (float)p->sponge_len;
+  q.sponge_len_m = len > 1.0f ? len : 1.0f;  // max(uSpongeLen, 1.0)
+  q.bl = (float)p->bl_thickness;
+  q.bl_m = q.bl > 1.0f ? q.bl : 1.0f;         // max(P.sponge.z, 1.0)
+  // Uniform sub-expressions, evaluated in float32 in the shader's order.
+  volatile float om = q.omega_plus;
+  q.tau0 = 1.0f / om;
+  volatile float t0 = q.tau0;
+  q.tau0sq = t0 * t0;
+  volatile float cs = q.les_cs;
+  volatile float c18 = 18.0f * cs;
+  q.c18cs2 = c18 * cs;
+  // Free-stream equilibrium of the sponge (ir.js:464-475), float32.
+  const float* ui = q.inlet;
+  volatile float uu_a = ui[0] * ui[0];
+  volatile float uu_b = uu_a + ui[1] * ui[1];
+  volatile float uuinf = uu_b + ui[2] * ui[2];
+  {
+    volatile float m15 = -1.5f * uuinf;
+    q.tinf[0] = wf(0) * m15;
+  }
+  for (int i = 1; i < Q; ++i) {
+    volatile float si = dotc_host(i, ui);
+    volatile float a = 3.0f * si;
+    volatile float b = 4.5f * si;
+    volatile float b2 = b * si;
+    volatile float ab = a + b2;
+    volatile float c = 1.5f * uuinf;
+    volatile float r = ab - c;
+    q.tinf[i] = wf(i) * r;
+  }
+  s->params = q;
+  s->configured = true;
+  return CFD_OK;
+}
+
+int cfd_set_types(cfd_solver* s, const uint32_t* tipo, size_t n) {
+  g_err.clear();
+  if (!s || !tipo) return fail(CFD_E_ARG, "cfd_set_types: null argument");
+  if (n != s->N)
+    return fail(CFD_E_ARG, "cfd_set_types: type field has %zu cells, expected %zu", n, s->N);
+  uint32_t* d32 = nullptr;
+  HIPCK(hipMalloc(&d32, n * sizeof(uint32_t)));
+  hipError_t e = hipMemcpyAsync(d32, tipo, n * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                s->stream);
+  if (e == hipSuccess) e = launch_prep_types(d32, s->tipo8, s->geom, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(d32);
+  if (e != hipSuccess) return fail(CFD_E_HIP, "cfd_set_types: %s", hipGetErrorString(e));
+  return CFD_OK;
+}
+
+int cfd_init_equilibrium(cfd_solver* s) {
+  g_err.clear();
+  if (!s) return fail(CFD_E_ARG, "cfd_init_equilibrium: null handle");
+  if (!s->configured)
+    return fail(CFD_E_STATE, "cfd_init_equilibrium: cfd_configure was never called");
+  HIPCK(launch_init(s->store, s->pop[1], s->tipo8, s->geom, s->params, s->stream));
+  s->front = 1;  // init writes B (solver.js:342-346)
+  return CFD_OK;
+}
+
+int cfd_set_populations(cfd_solver* s, const float* soa, size_t n) {
+  g_err.clear();
+  if (!s || !soa) return fail(CFD_E_ARG, "cfd_set_populations: null argument");
+  if (n != (size_t)Q * s->N)
+    return fail(CFD_E_ARG, "cfd_set_populations: %zu values, expected %zu", n,
+                (size_t)Q * s->N);
+  if (s->store == Store::F32) {
+    HIPCK(hipMemcpyAsync(s->pop[s->front], soa, n * sizeof(float), hipMemcpyHostToDevice,
+                         s->stream));
+    HIPCK(hipStreamSynchronize(s->stream));
+    return CFD_OK;
+  }
+  float* tmp = nullptr;
+  HIPCK(hipMalloc(&tmp, n * sizeof(float)));
+  hipError_t e = hipMemcpyAsync(tmp, soa, n * sizeof(float), hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = launch_f32_to_store(s->store, tmp, s->pop[s->front], n, s->N, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(CFD_E_HIP, "cfd_set_populations: %s", hipGetErrorString(e));
+  return CFD_OK;
+}
+
+int cfd_get_populations(cfd_solver* s, float* out, size_t n) {
+  g_err.clear();
+  if (!s || !out) return fail(CFD_E_ARG, "cfd_get_populations: null argument");
+  if (n != (size_t)Q * s->N)
+    return fail(CFD_E_ARG, "cfd_get_populations: %zu values, expected %zu", n,
+                (size_t)Q * s->N);
+  if (s->store == Store::F32) {
+    HIPCK(hipMemcpyAsync(out, s->pop[s->front], n * sizeof(float), hipMemcpyDeviceToHost,
+                         s->stream));
+    HIPCK(hipStreamSynchronize(s->stream));
+    return CFD_OK;
+  }
+  float* tmp = nullptr;
+  HIPCK(hipMalloc(&tmp, n * sizeof(float)));
+  hipError_t e = launch_store_to_f32(s->store, s->pop[s->front], tmp, n, s->N, s->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, tmp, n * sizeof(float), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(CFD_E_HIP, "cfd_get_populations: %s", hipGetErrorString(e));
+  return CFD_OK;
+}
+
+int cfd_set_front(cfd_solver* s, int front) {
+  g_err.clear();
+  if (!s) return fail(CFD_E_ARG, "cfd_set_front: null handle");
+  if (front != 0 && front != 1) return fail(CFD_E_ARG, "cfd_set_front: front must be 0 or 1");
+  s->front = front;
+  return CFD_OK;
+}
+
+int cfd_get_front(cfd_solver* s, int* front) {
+  g_err.clear();
+  if (!s || !front) return fail(CFD_E_ARG, "cfd_get_front: null argument");
+  *front = s->front;
+  return CFD_OK;
+}
+
+static int enqueue_steps(cfd_solver* s, int n) {
+  for (int k = 0; k < n; ++k) {
+    const int f = s->front;
+    HIPCK(launch_step(s->store, s->pop[f], s->pop[f ^ 1], s->tipo8, s->macros, k == n - 1,
+                      s->geom, s->params, s->stream));
+    s->front = f ^ 1;
+  }
+  return CFD_OK;
+}
+
+int cfd_step(cfd_solver* s, int n) {
+  g_err.clear();
+  if (!s) return fail(CFD_E_ARG, "cfd_step: null handle");
+  if (n < 0) return fail(CFD_E_ARG, "cfd_step: negative step count %d", n);
+  if (!s->configured) return fail(CFD_E_STATE, "cfd_step: cfd_configure was never called");
+  return enqueue_steps(s, n);
+}
+
+int cfd_step_timed(cfd_solver* s, int n, double* ms_out) {
+  g_err.clear();
+  if (!s || !ms_out) return fail(CFD_E_ARG, "cfd_step_timed: null argument");
+  if (n < 1) return fail(CFD_E_ARG, "cfd_step_timed: step count %d must be >= 1", n);
+  if (!s->configured)
+    return fail(CFD_E_STATE, "cfd_step_timed: cfd_configure was never called");
+  HIPCK(hipEventRecord(s->ev0, s->stream));
+  int rc = enqueue_steps(s, n);
+  if (rc) return rc;
+  HIPCK(hipEventRecord(s->ev1, s->stream));
+  HIPCK(hipEventSynchronize(s->ev1));
+  float ms = 0.f;
+  HIPCK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  *ms_out = ms;
+  return CFD_OK;
+}
+
+int cfd_forces(cfd_solver* s, double out_f[3]) {
+  g_err.clear();
+  if (!s || !out_f) return fail(CFD_E_ARG, "cfd_forces: null argument");
+  HIPCK(launch_forces(s->store, s->pop[s->front], s->tipo8, s->geom, s->partials, s->result,
+                      s->stream));
+  HIPCK(hipMemcpyAsync(s->host_res, s->result, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                       s->stream));
+  HIPCK(hipStreamSynchronize(s->stream));
+  for (int a = 0; a < 3; ++a) out_f[a] = s->host_res[a];
+  return CFD_OK;
+}
+
+int cfd_update_macros(cfd_solver* s) {
+  g_err.clear();
+  if (!s) return fail(CFD_E_ARG, "cfd_update_macros: null handle");
+  HIPCK(launch_macros(s->store, s->pop[s->front], s->macros, s->geom, s->stream));
+  return CFD_OK;
+}
+
+int cfd_read_macros(cfd_solver* s, float* out, size_t n) {
+  g_err.clear();
+  if (!s || !out) return fail(CFD_E_ARG, "cfd_read_macros: null argument");
+  if (n != 4 * s->N)
+    return fail(CFD_E_ARG, "cfd_read_macros: %zu values, expected %zu", n, 4 * s->N);
+  HIPCK(hipMemcpyAsync(out, s->macros, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+  HIPCK(hipStreamSynchronize(s->stream));
+  return CFD_OK;
+}
+
+int cfd_mass(cfd_solver* s, double* mass_out, int64_t* n_nonfinite_out) {
+  g_err.clear();
+  if (!s || !mass_out) return fail(CFD_E_ARG, "cfd_mass: null argument");
+  HIPCK(launch_mass(s->store, s->pop[s->front], s->tipo8, s->geom, s->partials, s->result,
+                    s->stream));
+  HIPCK(hipMemcpyAsync(s->host_res, s->result, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                       s->stream));
+  HIPCK(hipStreamSynchronize(s->stream));
+  *mass_out = s->host_res[0];
+  if (n_nonfinite_out) *n_nonfinite_out = (int64_t)s->host_res[1];
+  return CFD_OK;
+}
+
+int cfd_synchronize(cfd_solver* s) {
+  g_err.clear();
+  if (!s) return fail(CFD_E_ARG, "cfd_synchronize: null handle");
+  HIPCK(hipStreamSynchronize(s->stream));
+  return CFD_OK;
+}
+
+int cfd_bytes(cfd_solver* s, int64_t* bytes_out) {
+  g_err.clear();
+  if (!s || !bytes_out) return fail(CFD_E_ARG, "cfd_bytes: null argument");
+  *bytes_out = device_bytes(s);
+  return CFD_OK;
+}
+
+void cfd_destroy(cfd_solver* s) { release(s); }
+
+int cfd_copy_bandwidth(size_t bytes, int iters, double* gbps_out, double* ms_out) {
+  g_err.clear();
+  if (!gbps_out || bytes < 16 || iters < 1)
+    return fail(CFD_E_ARG, "cfd_copy_bandwidth: bad arguments");
+  const size_t n4 = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  HIPCK(hipEventCreate(&e0));
+  HIPCK(hipEventCreate(&e1));
+  hipError_t e = hipMalloc(&a, n4 * 16);
+  if (e == hipSuccess) e = hipMalloc(&b, n4 * 16);
+  if (e == hipSuccess) e = hipMemsetAsync(a, 0, n4 * 16, st);
+  if (e == hipSuccess) e = launch_copy(a, b, n4, st);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  for (int k = 0; k < iters && e == hipSuccess; ++k) e = launch_copy(a, b, n4, st);
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  if (e != hipSuccess) return fail(CFD_E_HIP, "cfd_copy_bandwidth: %s", hipGetErrorString(e));
+  const double per = ms / iters;
+  *gbps_out = 2.0 * (double)(n4 * 16) / (per * 1e-3) / 1e9;
+  if (ms_out) *ms_out = per;
+  return CFD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// Host-side launch interface of the device kernels.  Structs here are passed
+// by value as kernel arguments, so they stay PODs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace cfd {
+
+constexpr int kBlock = 256;  // threads per workgroup of the cell kernels
+constexpr int kXcds = 8;     // MI355X: workgroups go round-robin over 8 XCDs
+
+// Grid geometry plus the magic numbers of the two divisions that turn a
+// linear cell index into (x, y, z) without integer division instructions.
+struct Geom {
+  uint32_t nx, ny, nz, nxny, N;
+  uint32_t mnx, snx;    // n / nx   == (umulhi(n, mnx) + n) >> snx
+  uint32_t mnxy, snxy;  // n / nxny == (umulhi(n, mnxy) + n) >> snxy
+  uint32_t nblocks;     // logical workgroups covering N cells
+  uint32_t per_xcd;     // logical workgroups owned by each XCD
+};
+
+// Uniform values of one step, in float32 as the reference's Params uniform
+// holds them (solver.js:277-293), plus per-launch constants folded on the
+// host in exactly the float32 operation order the emitted shader uses.
+struct StepParams {
+  float omega_plus, magic, les_cs;
+  float tau0;       // 1.0 / omegaPlus        (ir.js:290)
+  float tau0sq;     // tau0 * tau0            (ir.js:292)
+  float c18cs2;     // 18.0 * Cs * Cs         (ir.js:293, left to right)
+  float belt[3];
+  float inlet[3];
+  float sponge_start, sponge_len_m;  // len_m = max(len, 1.0) (ir.js:461)
+  float bl, bl_m;                    // bl_m = max(bl, 1.0)   (wgsl.js:217)
+  float tinf[19];   // free-stream equilibrium of the sponge  (ir.js:464-475)
+};
+
+Geom make_geom(int nx, int ny, int nz);
+
+// Population storage: float32 (152 B/cell/step) or scaled float16 (76 B).
+enum class Store { F32 = 0, F16 = 1 };
+
+hipError_t launch_prep_types(const uint32_t* tipo32, uint8_t* tipo8, const Geom& g,
+                             hipStream_t st);
+hipError_t launch_step(Store sto, const void* src, void* dst, const uint8_t* tipo8,
+                       float4* macros, bool write_macros, const Geom& g,
+                       const StepParams& p, hipStream_t st);
+hipError_t launch_init(Store sto, void* dst, const uint8_t* tipo8, const Geom& g,
+                       const StepParams& p, hipStream_t st);
+hipError_t launch_macros(Store sto, const void* src, float4* macros, const Geom& g,
+                         hipStream_t st);
+// Momentum exchange (3 doubles) and mass (sum, non-finite count) reductions.
+// `partials` must hold kReduceGrid * 3 doubles; result lands in out[0..2].
+constexpr int kReduceGrid = 1024;
+hipError_t launch_forces(Store sto, const void* pop, const uint8_t* tipo8, const Geom& g,
+                         double* partials, double* out, hipStream_t st);
+hipError_t launch_mass(Store sto, const void* pop, const uint8_t* tipo8, const Geom& g,
+                       double* partials, double* out, hipStream_t st);
+hipError_t launch_f32_to_store(Store sto, const float* in, void* out, size_t n, size_t N,
+                               hipStream_t st);
+hipError_t launch_store_to_f32(Store sto, const void* in, float* out, size_t n, size_t N,
+                               hipStream_t st);
+hipError_t launch_copy(const float4* in, float4* out, size_t n4, hipStream_t st);
+
+}  // namespace cfd

@@ -1,0 +1,658 @@
+// D3Q19 TRT-LES kernels for gfx950 (MI355X).
+//
+// Physics follows the reference's IR (src/core/emit/ir.js) as emitted to WGSL
+// by src/core/emit/wgsl.js; citations are given per block.  The layout and
+// launch shape are this library's own:
+//
+//  * populations are SoA, pop[i*N + cell], cell = (z*ny + y)*nx + x, one
+//    buffer per population set (A/B ping-pong, solver.js:10-18);
+//  * one thread per cell over a flat index, 256-thread workgroups; the flat
+//    index is split into (x, y, z) with two multiply-high divisions;
+//  * workgroup IDs are remapped so that each of the 8 XCDs streams one
+//    contiguous slab of cells: neighbouring rows (y +- 1, z +- 1) pulled by a
+//    workgroup were written by workgroups of the same XCD, so the partial
+//    cache lines of the +-1 x-shifted pulls are shared inside one L2;
+//  * the cell type is one byte with two precomputed flags (near-wall, body
+//    link), so the 18 neighbour-type reads of the bounce-back test happen only
+//    for cells that touch a wall.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+#include "lattice.h"
+
+namespace cfd {
+
+// ------------------------------------------------------------------ storage
+
+struct StoreF32 {
+  using T = float;
+  static __device__ __forceinline__ float ld(const T* p) { return *p; }
+  static __device__ __forceinline__ void st(T* p, float v) { *p = v; }
+};
+
+// float16 deviations g_i = f_i - w_i scaled by 2^12 (exact power of two, so
+// the scaling itself adds no rounding): |g| up to 16 stays finite and values
+// down to 1.5e-8 stay normal.  Collision still runs in float32.
+struct StoreF16 {
+  using T = _Float16;
+  static __device__ __forceinline__ float ld(const T* p) {
+    return static_cast<float>(*p) * 0.000244140625f;
+  }
+  static __device__ __forceinline__ void st(T* p, float v) {
+    *p = static_cast<T>(v * 4096.0f);
+  }
+};
+
+// ------------------------------------------------------------------ indexing
+
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, uint32_t s) {
+  return (__umulhi(n, m) + n) >> s;
+}
+
+// Logical workgroup of this launch: physical workgroup b runs on XCD b % 8;
+// XCD k owns logical workgroups [k*per_xcd, (k+1)*per_xcd).
+__device__ __forceinline__ uint32_t logical_block(const Geom& g) {
+  const uint32_t b = blockIdx.x;
+  return (b % kXcds) * g.per_xcd + b / kXcds;
+}
+
+struct Coord {
+  uint32_t x, y, z;
+};
+
+__device__ __forceinline__ Coord split(uint32_t cell, const Geom& g) {
+  Coord c;
+  c.z = fast_div(cell, g.mnxy, g.snxy);
+  const uint32_t r = cell - c.z * g.nxny;
+  c.y = fast_div(r, g.mnx, g.snx);
+  c.x = r - c.y * g.nx;
+  return c;
+}
+
+// Signed index offsets to the periodic neighbours x-1, x+1 (and y, z), the
+// select-based wrap of wgsl.js:159-182 (no modulo).
+struct Wrap {
+  int xm, xp, ym, yp, zm, zp;
+};
+
+__device__ __forceinline__ Wrap wrap_offsets(const Coord& c, const Geom& g) {
+  Wrap w;
+  const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz, nxny = (int)g.nxny;
+  w.xm = c.x == 0 ? nx - 1 : -1;
+  w.xp = c.x == g.nx - 1 ? -(nx - 1) : 1;
+  w.ym = c.y == 0 ? (ny - 1) * nx : -nx;
+  w.yp = c.y == g.ny - 1 ? -(ny - 1) * nx : nx;
+  w.zm = c.z == 0 ? (nz - 1) * nxny : -nxny;
+  w.zp = c.z == g.nz - 1 ? -(nz - 1) * nxny : nxny;
+  return w;
+}
+
+// Offset of the cell at x + (dx, dy, dz), each component in {-1, 0, 1}.
+__device__ __forceinline__ int nb_offset(const Wrap& w, int dx, int dy, int dz) {
+  const int ox = dx < 0 ? w.xm : (dx > 0 ? w.xp : 0);
+  const int oy = dy < 0 ? w.ym : (dy > 0 ? w.yp : 0);
+  const int oz = dz < 0 ? w.zm : (dz > 0 ? w.zp : 0);
+  return ox + oy + oz;
+}
+
+// c_i . v written as the emitted literal arithmetic (ir.js:116-126): the
+// non-zero components in x, y, z order, a leading minus for a negative first
+// component, then + or -.
+__device__ __forceinline__ float dotc(int i, float vx, float vy, float vz) {
+  float acc = 0.0f;
+  bool first = true;
+  const int c[3] = {cx(i), cy(i), cz(i)};
+  const float v[3] = {vx, vy, vz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (c[a] == 0) continue;
+    if (first) {
+      acc = c[a] > 0 ? v[a] : -v[a];
+      first = false;
+    } else {
+      acc = c[a] > 0 ? acc + v[a] : acc - v[a];
+    }
+  }
+  return acc;
+}
+
+// ------------------------------------------------------------------ types
+
+// Pack the caller's u32 CELULA field into one byte per cell with the
+// near-wall and body-link flags (see lattice.h).
+__global__ __launch_bounds__(kBlock) void k_prep_types(const uint32_t* __restrict__ t32,
+                                                       uint8_t* __restrict__ t8, Geom g) {
+  const uint32_t cell = logical_block(g) * kBlock + threadIdx.x;
+  if (cell >= g.N) return;
+  const uint32_t t = t32[cell];
+  uint8_t code = t < 8u ? (uint8_t)t : (uint8_t)OUTRO;
+  const Coord c = split(cell, g);
+  const Wrap w = wrap_offsets(c, g);
+  bool near = false, body = false;
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    const uint32_t up = t32[(int)cell + nb_offset(w, -cx(i), -cy(i), -cz(i))];
+    near |= is_wall(up);
+    const uint32_t dn = t32[(int)cell + nb_offset(w, cx(i), cy(i), cz(i))];
+    body |= (dn == SOLIDO);
+  }
+  uint8_t out = code;
+  if (near) out |= NEAR_WALL;
+  if (body && code == FLUIDO) out |= BODY_LINK;
+  t8[cell] = out;
+}
+
+// ------------------------------------------------------------------ step
+
+// The fused hot kernel (wgsl.js:230-259, body = emitirPasso, ir.js:493-533):
+// pull streaming with halfway bounce-back at the fluid node, moments, then
+// the branch chain ESPELHO_Y -> ESPELHO_Z -> ENTRADA -> SAIDA -> fluid
+// (equilibrium, closed-form Smagorinsky, TRT per opposite pair, sponge).
+template <class S, bool MACROS>
+__global__ __launch_bounds__(kBlock) void k_step(const typename S::T* __restrict__ src,
+                                                 typename S::T* __restrict__ dst,
+                                                 const uint8_t* __restrict__ tipo,
+                                                 float4* __restrict__ macros, Geom g,
+                                                 StepParams p) {
+  using T = typename S::T;
+  const uint32_t cell = logical_block(g) * kBlock + threadIdx.x;
+  if (cell >= g.N) return;
+  const uint32_t tb = tipo[cell];
+  const uint32_t ct = tb & TYPE_MASK;
+
+  // Walls are not simulated: nothing ever reads their populations
+  // (wgsl.js:201-204); the macro field is zero there.
+  if (is_wall(ct)) {
+    if (MACROS) macros[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+
+  const Coord c = split(cell, g);
+  const Wrap w = wrap_offsets(c, g);
+  const size_t N = g.N;
+
+  // 1. pull (ir.js:166-189): g_i = src[i][x - c_i], or, when x - c_i is a
+  //    wall, the population this cell sent the other way last step plus the
+  //    moving-wall term 2 w_i (c_i . u_w) / c_s^2 with u_w = beltU on
+  //    SOLIDO_MOVEL and 0 elsewhere.
+  float gi[Q];
+  gi[0] = S::ld(src + cell);
+  if (!(tb & NEAR_WALL)) {
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const uint32_t s = (uint32_t)((int)cell + nb_offset(w, -cx(i), -cy(i), -cz(i)));
+      gi[i] = S::ld(src + (size_t)i * N + s);
+    }
+  } else {
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const uint32_t s = (uint32_t)((int)cell + nb_offset(w, -cx(i), -cy(i), -cz(i)));
+      const uint32_t t = tipo[s] & TYPE_MASK;
+      if (is_wall(t)) {
+        const bool moving = (t == SOLIDO_MOVEL);
+        const float cu = dotc(i, moving ? p.belt[0] : 0.f, moving ? p.belt[1] : 0.f,
+                              moving ? p.belt[2] : 0.f);
+        gi[i] = S::ld(src + (size_t)opp(i) * N + cell) + kwall(i) * cu;
+      } else {
+        gi[i] = S::ld(src + (size_t)i * N + s);
+      }
+    }
+  }
+
+  // 2. moments (ir.js:192-213), left-to-right sums as emitted.
+  float delta = gi[0];
+#pragma unroll
+  for (int i = 1; i < Q; ++i) delta = delta + gi[i];
+  const float rho = 1.0f + delta;
+  float m[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float acc = 0.0f;
+    bool first = true;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const int ca = a == 0 ? cx(i) : (a == 1 ? cy(i) : cz(i));
+      if (ca == 0) continue;
+      if (first) {
+        acc = ca > 0 ? gi[i] : -gi[i];
+        first = false;
+      } else {
+        acc = ca > 0 ? acc + gi[i] : acc - gi[i];
+      }
+    }
+    m[a] = acc;
+  }
+  const float ux = m[0] / rho, uy = m[1] / rho, uz = m[2] / rho;
+  const float uu = ux * ux + uy * uy + uz * uz;
+
+  float d[Q];
+  if (ct == ESPELHO_Y) {
+    // free-slip mirror (ir.js:383-390): reflect the y component
+#pragma unroll
+    for (int i = 0; i < Q; ++i) d[i] = gi[mirror(i, 1)];
+  } else if (ct == ESPELHO_Z) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) d[i] = gi[mirror(i, 2)];
+  } else if (ct == ENTRADA) {
+    // inlet (ir.js:405-422) with the 1/7-power profile (wgsl.js:217-220):
+    // equilibrium at the prescribed velocity, delta = 0, rho = 1.
+    const float zc = (float)c.z + 0.5f;
+    const float blf = zc < p.bl ? powf(zc / p.bl_m, 0.1428571429f) : 1.0f;
+    const float ex = p.inlet[0] * blf, ey = p.inlet[1] * blf, ez = p.inlet[2] * blf;
+    const float uuin = ex * ex + ey * ey + ez * ez;
+    d[0] = wf(0) * (0.0f - 1.5f * 1.0f * uuin);
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const float ci = dotc(i, ex, ey, ez);
+      d[i] = wf(i) * (0.0f + 1.0f * (3.0f * ci + 4.5f * ci * ci - 1.5f * uuin));
+    }
+  } else if (ct == SAIDA) {
+    // outlet (ir.js:432-444): equilibrium at the local u with delta = 0
+    d[0] = wf(0) * (-1.5f * uu);
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const float cu = dotc(i, ux, uy, uz);
+      d[i] = wf(i) * (3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
+    }
+  } else {
+    // fluid: equilibrium (ir.js:216-232)
+    float e[Q];
+    e[0] = wf(0) * (delta - 1.5f * rho * uu);
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const float cu = dotc(i, ux, uy, uz);
+      e[i] = wf(i) * (delta + rho * (3.0f * cu + 4.5f * cu * cu - 1.5f * uu));
+    }
+    // closed-form Smagorinsky with Lambda held fixed (ir.js:262-301)
+    float pab[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int a = k < 3 ? k : (k == 5 ? 1 : 0);
+      const int b = k < 3 ? k : (k == 3 ? 1 : 2);
+      float acc = 0.0f;
+      bool first = true;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const int ca = a == 0 ? cx(i) : (a == 1 ? cy(i) : cz(i));
+        const int cb = b == 0 ? cx(i) : (b == 1 ? cy(i) : cz(i));
+        const int coef = ca * cb;
+        if (coef == 0) continue;
+        const float n = gi[i] - e[i];
+        if (first) {
+          acc = coef > 0 ? n : -n;
+          first = false;
+        } else {
+          acc = coef > 0 ? acc + n : acc - n;
+        }
+      }
+      pab[k] = acc;
+    }
+    const float pp = pab[0] * pab[0] + pab[1] * pab[1] + pab[2] * pab[2] +
+                     2.0f * (pab[3] * pab[3] + pab[4] * pab[4] + pab[5] * pab[5]);
+    const float qs = sqrtf(2.0f * pp);
+    const float tauT = 0.5f * (p.tau0 + sqrtf(p.tau0sq + p.c18cs2 * qs / rho));
+    const float omP = 1.0f / tauT;
+    const float omM = 1.0f / (p.magic / (1.0f / omP - 0.5f) + 0.5f);
+    // TRT collision per opposite pair (ir.js:304-323)
+    d[0] = gi[0] - omP * (gi[0] - e[0]);
+#pragma unroll
+    for (int i = 1; i < Q; i += 2) {
+      const int j = i + 1;
+      const float pe = 0.5f * ((gi[i] + gi[j]) - (e[i] + e[j]));
+      const float mo = 0.5f * ((gi[i] - gi[j]) - (e[i] - e[j]));
+      d[i] = gi[i] - omP * pe - omM * mo;
+      d[j] = gi[j] - omP * pe + omM * mo;
+    }
+    // sponge (ir.js:458-478): mix(dst, teq, sigma^2).  For x <= start
+    // sigma^2 = 0 and mix returns dst unchanged, so the blend is skipped.
+    const float dxs = (float)c.x - p.sponge_start;
+    if (dxs > 0.0f) {
+      const float sigma = fminf(dxs / p.sponge_len_m, 1.0f);
+      const float sig2 = sigma * sigma;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) d[i] = d[i] * (1.0f - sig2) + p.tinf[i] * sig2;
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < Q; ++i) S::st(dst + (size_t)i * N + cell, d[i]);
+  if (MACROS) macros[cell] = make_float4(ux, uy, uz, delta);
+}
+
+// ------------------------------------------------------------------ init
+
+// Free-stream equilibrium everywhere (wgsl.js:269-302): u = inletU, 0 in
+// SOLIDO, beltU in SOLIDO_MOVEL (PAREDE keeps inletU, as in the reference).
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_init(typename S::T* __restrict__ dst,
+                                                 const uint8_t* __restrict__ tipo, Geom g,
+                                                 StepParams p) {
+  const uint32_t cell = logical_block(g) * kBlock + threadIdx.x;
+  if (cell >= g.N) return;
+  const uint32_t ct = tipo[cell] & TYPE_MASK;
+  float ux = p.inlet[0], uy = p.inlet[1], uz = p.inlet[2];
+  if (ct == SOLIDO) ux = uy = uz = 0.0f;
+  if (ct == SOLIDO_MOVEL) {
+    ux = p.belt[0];
+    uy = p.belt[1];
+    uz = p.belt[2];
+  }
+  const float uu = ux * ux + uy * uy + uz * uz;
+  const size_t N = g.N;
+  S::st(dst + cell, wf(0) * (0.0f - 1.5f * 1.0f * uu));
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    const float cu = dotc(i, ux, uy, uz);
+    S::st(dst + (size_t)i * N + cell,
+          wf(i) * (0.0f + 1.0f * (3.0f * cu + 4.5f * cu * cu - 1.5f * uu)));
+  }
+}
+
+// ------------------------------------------------------------------ macros
+
+// Moments of the stored populations without streaming (wgsl.js:442-476).
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_macros(const typename S::T* __restrict__ src,
+                                                   float4* __restrict__ macros, Geom g) {
+  const uint32_t cell = logical_block(g) * kBlock + threadIdx.x;
+  if (cell >= g.N) return;
+  const size_t N = g.N;
+  float gi[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) gi[i] = S::ld(src + (size_t)i * N + cell);
+  float delta = gi[0];
+#pragma unroll
+  for (int i = 1; i < Q; ++i) delta = delta + gi[i];
+  const float rho = 1.0f + delta;
+  float m[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float acc = 0.0f;
+    bool first = true;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const int ca = a == 0 ? cx(i) : (a == 1 ? cy(i) : cz(i));
+      if (ca == 0) continue;
+      if (first) {
+        acc = ca > 0 ? gi[i] : -gi[i];
+        first = false;
+      } else {
+        acc = ca > 0 ? acc + gi[i] : acc - gi[i];
+      }
+    }
+    m[a] = acc;
+  }
+  macros[cell] = make_float4(m[0] / rho, m[1] / rho, m[2] / rho, delta);
+}
+
+// ------------------------------------------------------------------ reductions
+
+// Wave64 butterfly sum, then the four waves of the workgroup through LDS in
+// fixed order: the result does not depend on scheduling.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* out) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  }
+  __shared__ double sh[kBlock / 64][NV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[wave][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = sh[0][k];
+      for (int q = 1; q < kBlock / 64; ++q) s += sh[q][k];
+      out[k] = s;
+    }
+  }
+}
+
+// Momentum exchange (wgsl.js:332-396): for a FLUIDO cell x and a SOLIDO
+// neighbour at x + c_i, the link delivers c_i * 2 (g_i(x) + w_i) to the body
+// (the 2 w_i c_i term does not cancel per link: lattice.js:37-45).  Only
+// cells flagged BODY_LINK look at their neighbours; accumulation is float64.
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_forces(const typename S::T* __restrict__ pop,
+                                                   const uint8_t* __restrict__ tipo, Geom g,
+                                                   double* __restrict__ partials) {
+  const size_t N = g.N;
+  double f[3] = {0.0, 0.0, 0.0};
+  for (uint32_t cell = blockIdx.x * kBlock + threadIdx.x; cell < g.N;
+       cell += gridDim.x * kBlock) {
+    if (!(tipo[cell] & BODY_LINK)) continue;
+    const Coord c = split(cell, g);
+    const Wrap w = wrap_offsets(c, g);
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const uint32_t nb = (uint32_t)((int)cell + nb_offset(w, cx(i), cy(i), cz(i)));
+      if ((tipo[nb] & TYPE_MASK) != SOLIDO) continue;
+      const double q = 2.0 * ((double)S::ld(pop + (size_t)i * N + cell) + wd(i));
+      if (cx(i)) f[0] += cx(i) > 0 ? q : -q;
+      if (cy(i)) f[1] += cy(i) > 0 ? q : -q;
+      if (cz(i)) f[2] += cz(i) > 0 ? q : -q;
+    }
+  }
+  block_sum<3>(f, partials + (size_t)blockIdx.x * 3);
+}
+
+// Mass: float64 sum of delta = sum_i g_i over non-wall cells, and a count of
+// cells holding a non-finite population.
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_mass(const typename S::T* __restrict__ pop,
+                                                 const uint8_t* __restrict__ tipo, Geom g,
+                                                 double* __restrict__ partials) {
+  const size_t N = g.N;
+  double v[2] = {0.0, 0.0};
+  for (uint32_t cell = blockIdx.x * kBlock + threadIdx.x; cell < g.N;
+       cell += gridDim.x * kBlock) {
+    if (is_wall(tipo[cell] & TYPE_MASK)) continue;
+    double s = 0.0;
+    bool bad = false;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const float x = S::ld(pop + (size_t)i * N + cell);
+      bad |= !isfinite(x);
+      s += (double)x;
+    }
+    if (bad) {
+      v[1] += 1.0;
+    } else {
+      v[0] += s;
+    }
+  }
+  block_sum<2>(v, partials + (size_t)blockIdx.x * 2);
+}
+
+// Second stage: one workgroup adds the kReduceGrid partials in a fixed order.
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_reduce(const double* __restrict__ partials,
+                                                   int n, double* __restrict__ out) {
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.0;
+  for (int j = threadIdx.x; j < n; j += kBlock) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += partials[(size_t)j * NV + k];
+  }
+  block_sum<NV>(v, out);
+}
+
+// ------------------------------------------------------------------ layout
+
+template <class S>
+__global__ void k_from_f32(const float* __restrict__ in, typename S::T* __restrict__ out,
+                           size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    S::st(out + k, in[k]);
+}
+template <class S>
+__global__ void k_to_f32(const typename S::T* __restrict__ in, float* __restrict__ out,
+                         size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    out[k] = S::ld(in + k);
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(const float4* __restrict__ in,
+                                                 float4* __restrict__ out, size_t n4) {
+  for (size_t k = blockIdx.x * (size_t)kBlock + threadIdx.x; k < n4;
+       k += (size_t)gridDim.x * kBlock)
+    out[k] = in[k];
+}
+
+// ------------------------------------------------------------------ host side
+
+// Round-up multiplier for n / d with n < 2^31 (Granlund & Montgomery 1994,
+// section 4): m = floor(2^32 (2^l - d) / d) + 1 with l = ceil(log2 d); then
+// n / d = (umulhi(n, m) + n) >> l.
+static void magic_u32(uint32_t d, uint32_t* m, uint32_t* s) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  const uint64_t num = (1ull << 32) * ((1ull << l) - d);
+  *m = (uint32_t)(num / d + 1);
+  *s = l;
+}
+
+Geom make_geom(int nx, int ny, int nz) {
+  Geom g;
+  g.nx = (uint32_t)nx;
+  g.ny = (uint32_t)ny;
+  g.nz = (uint32_t)nz;
+  g.nxny = g.nx * g.ny;
+  g.N = g.nxny * g.nz;
+  magic_u32(g.nx, &g.mnx, &g.snx);
+  magic_u32(g.nxny, &g.mnxy, &g.snxy);
+  g.nblocks = (g.N + kBlock - 1) / kBlock;
+  g.per_xcd = (g.nblocks + kXcds - 1) / kXcds;
+  return g;
+}
+
+static inline dim3 cell_grid(const Geom& g) { return dim3(g.per_xcd * kXcds); }
+
+hipError_t launch_prep_types(const uint32_t* t32, uint8_t* t8, const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_prep_types, cell_grid(g), dim3(kBlock), 0, st, t32, t8, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_step(Store sto, const void* src, void* dst, const uint8_t* tipo8,
+                       float4* macros, bool write_macros, const Geom& g,
+                       const StepParams& p, hipStream_t st) {
+  const dim3 grid = cell_grid(g), blk(kBlock);
+  if (sto == Store::F32) {
+    using S = StoreF32;
+    auto s = static_cast<const S::T*>(src);
+    auto d = static_cast<S::T*>(dst);
+    if (write_macros)
+      hipLaunchKernelGGL((k_step<S, true>), grid, blk, 0, st, s, d, tipo8, macros, g, p);
+    else
+      hipLaunchKernelGGL((k_step<S, false>), grid, blk, 0, st, s, d, tipo8, macros, g, p);
+  } else {
+    using S = StoreF16;
+    auto s = static_cast<const S::T*>(src);
+    auto d = static_cast<S::T*>(dst);
+    if (write_macros)
+      hipLaunchKernelGGL((k_step<S, true>), grid, blk, 0, st, s, d, tipo8, macros, g, p);
+    else
+      hipLaunchKernelGGL((k_step<S, false>), grid, blk, 0, st, s, d, tipo8, macros, g, p);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_init(Store sto, void* dst, const uint8_t* tipo8, const Geom& g,
+                       const StepParams& p, hipStream_t st) {
+  if (sto == Store::F32)
+    hipLaunchKernelGGL(k_init<StoreF32>, cell_grid(g), dim3(kBlock), 0, st,
+                       static_cast<float*>(dst), tipo8, g, p);
+  else
+    hipLaunchKernelGGL(k_init<StoreF16>, cell_grid(g), dim3(kBlock), 0, st,
+                       static_cast<_Float16*>(dst), tipo8, g, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_macros(Store sto, const void* src, float4* macros, const Geom& g,
+                         hipStream_t st) {
+  if (sto == Store::F32)
+    hipLaunchKernelGGL(k_macros<StoreF32>, cell_grid(g), dim3(kBlock), 0, st,
+                       static_cast<const float*>(src), macros, g);
+  else
+    hipLaunchKernelGGL(k_macros<StoreF16>, cell_grid(g), dim3(kBlock), 0, st,
+                       static_cast<const _Float16*>(src), macros, g);
+  return hipGetLastError();
+}
+
+static int reduce_grid(const Geom& g) {
+  const uint32_t b = (g.N + kBlock - 1) / kBlock;
+  return b < (uint32_t)kReduceGrid ? (int)b : kReduceGrid;
+}
+
+hipError_t launch_forces(Store sto, const void* pop, const uint8_t* tipo8, const Geom& g,
+                         double* partials, double* out, hipStream_t st) {
+  const int gb = reduce_grid(g);
+  if (sto == Store::F32)
+    hipLaunchKernelGGL(k_forces<StoreF32>, dim3(gb), dim3(kBlock), 0, st,
+                       static_cast<const float*>(pop), tipo8, g, partials);
+  else
+    hipLaunchKernelGGL(k_forces<StoreF16>, dim3(gb), dim3(kBlock), 0, st,
+                       static_cast<const _Float16*>(pop), tipo8, g, partials);
+  hipLaunchKernelGGL(k_reduce<3>, dim3(1), dim3(kBlock), 0, st, partials, gb, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mass(Store sto, const void* pop, const uint8_t* tipo8, const Geom& g,
+                       double* partials, double* out, hipStream_t st) {
+  const int gb = reduce_grid(g);
+  if (sto == Store::F32)
+    hipLaunchKernelGGL(k_mass<StoreF32>, dim3(gb), dim3(kBlock), 0, st,
+                       static_cast<const float*>(pop), tipo8, g, partials);
+  else
+    hipLaunchKernelGGL(k_mass<StoreF16>, dim3(gb), dim3(kBlock), 0, st,
+                       static_cast<const _Float16*>(pop), tipo8, g, partials);
+  hipLaunchKernelGGL(k_reduce<2>, dim3(1), dim3(kBlock), 0, st, partials, gb, out);
+  return hipGetLastError();
+}
+
+static dim3 flat_grid(size_t n) {
+  size_t b = (n + kBlock - 1) / kBlock;
+  if (b > 65536) b = 65536;
+  if (b == 0) b = 1;
+  return dim3((unsigned)b);
+}
+
+hipError_t launch_f32_to_store(Store sto, const float* in, void* out, size_t n, size_t,
+                               hipStream_t st) {
+  if (sto == Store::F32)
+    hipLaunchKernelGGL(k_from_f32<StoreF32>, flat_grid(n), dim3(kBlock), 0, st, in,
+                       static_cast<float*>(out), n);
+  else
+    hipLaunchKernelGGL(k_from_f32<StoreF16>, flat_grid(n), dim3(kBlock), 0, st, in,
+                       static_cast<_Float16*>(out), n);
+  return hipGetLastError();
+}
+
+hipError_t launch_store_to_f32(Store sto, const void* in, float* out, size_t n, size_t,
+                               hipStream_t st) {
+  if (sto == Store::F32)
+    hipLaunchKernelGGL(k_to_f32<StoreF32>, flat_grid(n), dim3(kBlock), 0, st,
+                       static_cast<const float*>(in), out, n);
+  else
+    hipLaunchKernelGGL(k_to_f32<StoreF16>, flat_grid(n), dim3(kBlock), 0, st,
+                       static_cast<const _Float16*>(in), out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const float4* in, float4* out, size_t n4, hipStream_t st) {
+  size_t b = (n4 + kBlock - 1) / kBlock;
+  if (b > 8192) b = 8192;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)b), dim3(kBlock), 0, st, in, out, n4);
+  return hipGetLastError();
+}
+
+}  // namespace cfd

@@ -1,0 +1,504 @@
+// GPU voxelizer, bit-identical to voxelizar() of src/geom/voxel.js.
+//
+// This file MUST be compiled with -ffp-contract=off: the reference computes
+// in IEEE float64 (JavaScript numbers) with every product rounded before the
+// following add, and the shell classification compares those results against
+// 0.75 and against the separating-axis radii.  Division and sqrt are the
+// correctly rounded float64 operations.
+//
+// Stages (each one a kernel over cells or triangles):
+//   raster   one workgroup per triangle walks the triangle's padded box; per
+//            cell: float64 point-triangle distance (voxel.js:105-151), the
+//            running minimum kept as float32 bits with atomicMin (the minimum
+//            of rounded values equals the rounded minimum, so the order of
+//            triangles cannot change it), and the exact triangle/box overlap
+//            (voxel.js:61-98) for cells within sqrt(0.75) -> CASCA.
+//   dilate   26-neighbour dilation of the shell, the flood barrier (244-267).
+//   flood    6-connected reachability from the six faces (278-327), computed
+//            as a monotone fixpoint: each workgroup loads a 32x8x4 tile plus
+//            halo into LDS and floods it to local convergence; tiles repeat
+//            until no cell changes anywhere.  Reachability is unique, so the
+//            result equals the reference's depth-first search.
+//   skin     one 6-neighbour growth of the exterior from a snapshot, not
+//            through original shell cells (333-355).
+//   final    tipo = CASCA / DENTRO / FLUIDO, counts, and the signed distance.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/cfd_mi355x.h"
+
+namespace {
+
+constexpr uint8_t CASCA = 1;
+constexpr uint32_t INF_BITS = 0x7f800000u;  // +Infinity as float32 bits
+
+__device__ __forceinline__ double dmin3(double a, double b, double c) {
+  double m = a < b ? a : b;
+  return m < c ? m : c;
+}
+__device__ __forceinline__ double dmax3(double a, double b, double c) {
+  double m = a > b ? a : b;
+  return m > c ? m : c;
+}
+
+// Akenine-Moller triangle / AABB overlap with the box centred at the origin
+// and half-size h on every axis (voxel.js:61-98, same operation order).
+__device__ bool tri_box(const double v0[3], const double v1[3], const double v2[3], double h) {
+  const double e0[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+  const double e1[3] = {v2[0] - v1[0], v2[1] - v1[1], v2[2] - v1[2]};
+  const double e2[3] = {v0[0] - v2[0], v0[1] - v2[1], v0[2] - v2[2]};
+  const double* es[3] = {e0, e1, e2};
+  for (int k = 0; k < 3; ++k) {
+    const double* e = es[k];
+    const double fx = fabs(e[0]), fy = fabs(e[1]), fz = fabs(e[2]);
+    // axis (a, b) applied to components (i, j), radius fa*h + fb*h
+    {
+      const double a = e[2], b = -e[1];
+      const double p0 = a * v0[1] + b * v0[2];
+      const double p1 = a * v1[1] + b * v1[2];
+      const double p2 = a * v2[1] + b * v2[2];
+      const double r = fz * h + fy * h;
+      if (dmin3(p0, p1, p2) > r || dmax3(p0, p1, p2) < -r) return false;
+    }
+    {
+      const double a = -e[2], b = e[0];
+      const double p0 = a * v0[0] + b * v0[2];
+      const double p1 = a * v1[0] + b * v1[2];
+      const double p2 = a * v2[0] + b * v2[2];
+      const double r = fz * h + fx * h;
+      if (dmin3(p0, p1, p2) > r || dmax3(p0, p1, p2) < -r) return false;
+    }
+    {
+      const double a = e[1], b = -e[0];
+      const double p0 = a * v0[0] + b * v0[1];
+      const double p1 = a * v1[0] + b * v1[1];
+      const double p2 = a * v2[0] + b * v2[1];
+      const double r = fy * h + fx * h;
+      if (dmin3(p0, p1, p2) > r || dmax3(p0, p1, p2) < -r) return false;
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    if (dmin3(v0[k], v1[k], v2[k]) > h) return false;
+    if (dmax3(v0[k], v1[k], v2[k]) < -h) return false;
+  }
+  const double n0 = e0[1] * e1[2] - e0[2] * e1[1];
+  const double n1 = e0[2] * e1[0] - e0[0] * e1[2];
+  const double n2 = e0[0] * e1[1] - e0[1] * e1[0];
+  const double d = n0 * v0[0] + n1 * v0[1] + n2 * v0[2];
+  const double r = fabs(n0) * h + fabs(n1) * h + fabs(n2) * h;
+  return fabs(d) <= r;
+}
+
+// Squared point-triangle distance over the seven Voronoi regions (Ericson,
+// Real-Time Collision Detection 5.1.5), voxel.js:105-151 operation by operation.
+__device__ double dist2_point_tri(const double p[3], const double a[3], const double b[3],
+                                  const double c[3]) {
+  const double abx = b[0] - a[0], aby = b[1] - a[1], abz = b[2] - a[2];
+  const double acx = c[0] - a[0], acy = c[1] - a[1], acz = c[2] - a[2];
+  const double apx = p[0] - a[0], apy = p[1] - a[1], apz = p[2] - a[2];
+  const double d1 = abx * apx + aby * apy + abz * apz;
+  const double d2 = acx * apx + acy * apy + acz * apz;
+  if (d1 <= 0 && d2 <= 0) return apx * apx + apy * apy + apz * apz;
+
+  const double bpx = p[0] - b[0], bpy = p[1] - b[1], bpz = p[2] - b[2];
+  const double d3 = abx * bpx + aby * bpy + abz * bpz;
+  const double d4 = acx * bpx + acy * bpy + acz * bpz;
+  if (d3 >= 0 && d4 <= d3) return bpx * bpx + bpy * bpy + bpz * bpz;
+
+  const double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+    const double v = d1 / (d1 - d3);
+    const double qx = apx - v * abx, qy = apy - v * aby, qz = apz - v * abz;
+    return qx * qx + qy * qy + qz * qz;
+  }
+
+  const double cpx = p[0] - c[0], cpy = p[1] - c[1], cpz = p[2] - c[2];
+  const double d5 = abx * cpx + aby * cpy + abz * cpz;
+  const double d6 = acx * cpx + acy * cpy + acz * cpz;
+  if (d6 >= 0 && d5 <= d6) return cpx * cpx + cpy * cpy + cpz * cpz;
+
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+    const double w = d2 / (d2 - d6);
+    const double qx = apx - w * acx, qy = apy - w * acy, qz = apz - w * acz;
+    return qx * qx + qy * qy + qz * qz;
+  }
+
+  const double va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    const double qx = bpx + w * (cpx - bpx), qy = bpy + w * (cpy - bpy),
+                 qz = bpz + w * (cpz - bpz);
+    return qx * qx + qy * qy + qz * qz;
+  }
+
+  const double den = 1 / (va + vb + vc);
+  const double v = vb * den, w = vc * den;
+  const double qx = apx - (v * abx + w * acx);
+  const double qy = apy - (v * aby + w * acy);
+  const double qz = apz - (v * abz + w * acz);
+  return qx * qx + qy * qy + qz * qz;
+}
+
+constexpr int kRasterBlock = 64;
+
+__global__ __launch_bounds__(kRasterBlock) void k_raster(
+    const float* __restrict__ pos, size_t nv, const uint32_t* __restrict__ idx, size_t nt,
+    int nx, int ny, int nz, double band, uint8_t* __restrict__ flags,
+    uint32_t* __restrict__ dist2bits, unsigned long long* __restrict__ outside) {
+  for (size_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    const size_t i0 = idx[t * 3], i1 = idx[t * 3 + 1], i2 = idx[t * 3 + 2];
+    // An index past the vertex array reads `undefined` in the reference,
+    // which poisons the finiteness sum below: the triangle is skipped.
+    if (i0 >= nv || i1 >= nv || i2 >= nv) continue;
+    const double a[3] = {pos[i0 * 3], pos[i0 * 3 + 1], pos[i0 * 3 + 2]};
+    const double b[3] = {pos[i1 * 3], pos[i1 * 3 + 1], pos[i1 * 3 + 2]};
+    const double c[3] = {pos[i2 * 3], pos[i2 * 3 + 1], pos[i2 * 3 + 2]};
+    const double sum = a[0] + a[1] + a[2] + b[0] + b[1] + b[2] + c[0] + c[1] + c[2];
+    if (!isfinite(sum)) continue;
+
+    // padded triangle box clipped to the grid (voxel.js:195-201)
+    double lo[3], hi[3];
+    const int n[3] = {nx, ny, nz};
+    bool empty = false;
+    for (int k = 0; k < 3; ++k) {
+      const double f = floor(dmin3(a[k], b[k], c[k]) - band);
+      lo[k] = f > 0.0 ? f : 0.0;
+      const double g = ceil(dmax3(a[k], b[k], c[k]) + band);
+      hi[k] = g < (double)(n[k] - 1) ? g : (double)(n[k] - 1);
+      if (hi[k] < lo[k]) empty = true;
+    }
+    if (empty) {
+      if (threadIdx.x == 0) atomicAdd(outside, 1ull);
+      continue;
+    }
+    const int x0 = (int)lo[0], y0 = (int)lo[1], z0 = (int)lo[2];
+    const int bx = (int)hi[0] - x0 + 1, by = (int)hi[1] - y0 + 1, bz = (int)hi[2] - z0 + 1;
+    const size_t ncell = (size_t)bx * by * bz;
+    for (size_t k = threadIdx.x; k < ncell; k += kRasterBlock) {
+      const int x = x0 + (int)(k % bx);
+      const size_t r = k / bx;
+      const int y = y0 + (int)(r % by);
+      const int z = z0 + (int)(r / by);
+      const double p[3] = {x + 0.5, y + 0.5, z + 0.5};
+      const size_t cell = ((size_t)z * ny + y) * nx + x;
+      const double d2 = dist2_point_tri(p, a, b, c);
+      if (d2 == d2) atomicMin(&dist2bits[cell], __float_as_uint((float)d2));
+      if (d2 <= 0.75 && flags[cell] != CASCA) {
+        const double la[3] = {a[0] - p[0], a[1] - p[1], a[2] - p[2]};
+        const double lb[3] = {b[0] - p[0], b[1] - p[1], b[2] - p[2]};
+        const double lc[3] = {c[0] - p[0], c[1] - p[1], c[2] - p[2]};
+        if (tri_box(la, lb, lc, 0.5)) flags[cell] = CASCA;
+      }
+    }
+  }
+}
+
+constexpr int kCellBlock = 256;
+
+// 26-neighbour dilation of the shell, clamped at the faces (voxel.js:244-267).
+__global__ __launch_bounds__(kCellBlock) void k_dilate(const uint8_t* __restrict__ flags,
+                                                       uint8_t* __restrict__ barrier, int nx,
+                                                       int ny, int nz) {
+  const size_t N = (size_t)nx * ny * nz;
+  const size_t cell = blockIdx.x * (size_t)kCellBlock + threadIdx.x;
+  if (cell >= N) return;
+  const int x = (int)(cell % nx);
+  const int y = (int)((cell / nx) % ny);
+  const int z = (int)(cell / ((size_t)nx * ny));
+  uint8_t hit = 0;
+  for (int k = z > 0 ? z - 1 : 0; k <= (z < nz - 1 ? z + 1 : nz - 1); ++k)
+    for (int j = y > 0 ? y - 1 : 0; j <= (y < ny - 1 ? y + 1 : ny - 1); ++j)
+      for (int i = x > 0 ? x - 1 : 0; i <= (x < nx - 1 ? x + 1 : nx - 1); ++i)
+        hit |= (flags[((size_t)k * ny + j) * nx + i] == CASCA);
+  barrier[cell] = hit;
+}
+
+// Seed: every face cell that is not a barrier is exterior (voxel.js:291-309).
+__global__ __launch_bounds__(kCellBlock) void k_seed(const uint8_t* __restrict__ barrier,
+                                                     uint8_t* __restrict__ ext, int nx, int ny,
+                                                     int nz) {
+  const size_t N = (size_t)nx * ny * nz;
+  const size_t cell = blockIdx.x * (size_t)kCellBlock + threadIdx.x;
+  if (cell >= N) return;
+  const int x = (int)(cell % nx);
+  const int y = (int)((cell / nx) % ny);
+  const int z = (int)(cell / ((size_t)nx * ny));
+  const bool face = x == 0 || x == nx - 1 || y == 0 || y == ny - 1 || z == 0 || z == nz - 1;
+  ext[cell] = (face && !barrier[cell]) ? 1 : 0;
+}
+
+// Tile flood.  LDS state per cell: 0 open, 1 exterior, 2 blocked (barrier or
+// outside the grid).  Interior cells are updated in place until the tile
+// stops changing; halo cells are the global state at load time.
+constexpr int TX = 32, TY = 8, TZ = 4;
+constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+
+__global__ __launch_bounds__(kCellBlock) void k_flood_tile(const uint8_t* __restrict__ barrier,
+                                                           uint8_t* __restrict__ ext, int nx,
+                                                           int ny, int nz,
+                                                           int* __restrict__ changed) {
+  __shared__ uint8_t st[HZ][HY][HX];
+  const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
+  const int bxi = blockIdx.x % gx;
+  const int byi = (blockIdx.x / gx) % gy;
+  const int bzi = blockIdx.x / (gx * gy);
+  const int ox = bxi * TX - 1, oy = byi * TY - 1, oz = bzi * TZ - 1;
+  for (int k = threadIdx.x; k < HX * HY * HZ; k += kCellBlock) {
+    const int lx = k % HX, ly = (k / HX) % HY, lz = k / (HX * HY);
+    const int x = ox + lx, y = oy + ly, z = oz + lz;
+    uint8_t v = 2;
+    if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
+      const size_t cell = ((size_t)z * ny + y) * nx + x;
+      v = barrier[cell] ? 2 : (ext[cell] ? 1 : 0);
+    }
+    st[lz][ly][lx] = v;
+  }
+  __syncthreads();
+  // each thread owns the 4 cells (lx, ly, 1..4) of its column
+  const int lx = 1 + threadIdx.x % TX, ly = 1 + threadIdx.x / TX;
+  uint8_t owned_new = 0;  // bit z: cell became exterior in this launch
+  for (;;) {
+    int mine = 0;
+    for (int lz = 1; lz <= TZ; ++lz) {
+      if (st[lz][ly][lx] != 0) continue;
+      if (st[lz][ly][lx - 1] == 1 || st[lz][ly][lx + 1] == 1 || st[lz][ly - 1][lx] == 1 ||
+          st[lz][ly + 1][lx] == 1 || st[lz - 1][ly][lx] == 1 || st[lz + 1][ly][lx] == 1) {
+        st[lz][ly][lx] = 1;
+        owned_new |= (uint8_t)(1u << lz);
+        mine = 1;
+      }
+    }
+    // barrier + OR: every thread sees the same decision, and the next sweep
+    // starts only after every write of this one
+    if (!__syncthreads_or(mine)) break;
+  }
+  if (owned_new) {
+    const int x = ox + lx, y = oy + ly;
+    for (int lz = 1; lz <= TZ; ++lz) {
+      if (!(owned_new & (1u << lz))) continue;
+      const int z = oz + lz;
+      ext[((size_t)z * ny + y) * nx + x] = 1;
+    }
+    *changed = 1;
+  }
+}
+
+// Skin recovery from a snapshot (voxel.js:333-355).
+__global__ __launch_bounds__(kCellBlock) void k_skin(const uint8_t* __restrict__ before,
+                                                     const uint8_t* __restrict__ flags,
+                                                     uint8_t* __restrict__ after, int nx,
+                                                     int ny, int nz) {
+  const size_t N = (size_t)nx * ny * nz;
+  const size_t c = blockIdx.x * (size_t)kCellBlock + threadIdx.x;
+  if (c >= N) return;
+  const int x = (int)(c % nx);
+  const int y = (int)((c / nx) % ny);
+  const int z = (int)(c / ((size_t)nx * ny));
+  const size_t nxny = (size_t)nx * ny;
+  uint8_t v = before[c];
+  if (!v && flags[c] != CASCA) {
+    v = (x > 0 && before[c - 1]) || (x < nx - 1 && before[c + 1]) ||
+        (y > 0 && before[c - nx]) || (y < ny - 1 && before[c + nx]) ||
+        (z > 0 && before[c - nxny]) || (z < nz - 1 && before[c + nxny]);
+  }
+  after[c] = v;
+}
+
+// tipo, counts and signed distance (voxel.js:401-415).
+__global__ __launch_bounds__(kCellBlock) void k_final(
+    const uint8_t* __restrict__ flags, const uint8_t* __restrict__ ext,
+    const uint32_t* __restrict__ dist2bits, size_t N, double band, uint8_t* __restrict__ tipo,
+    float* __restrict__ sdf, unsigned long long* __restrict__ counts) {
+  const size_t c = blockIdx.x * (size_t)kCellBlock + threadIdx.x;
+  unsigned long long casca = 0, dentro = 0;
+  if (c < N) {
+    uint8_t t = 0;
+    if (flags[c] == CASCA) {
+      t = 1;
+      casca = 1;
+    } else if (!ext[c]) {
+      t = 2;
+      dentro = 1;
+    }
+    tipo[c] = t;
+    if (sdf) {
+      const uint32_t bits = dist2bits[c];
+      const double d = bits == INF_BITS ? band + 1 : sqrt((double)__uint_as_float(bits));
+      sdf[c] = (float)(t == 0 ? d : -d);
+    }
+  }
+  // wave-aggregated counters
+  for (int off = 32; off > 0; off >>= 1) {
+    casca += __shfl_xor(casca, off, 64);
+    dentro += __shfl_xor(dentro, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (casca) atomicAdd(&counts[0], casca);
+    if (dentro) atomicAdd(&counts[1], dentro);
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    p[k] = v;
+}
+
+}  // namespace
+
+// The solver translation unit owns cfd_last_error(); the voxelizer reports
+// through it by writing the shared message with this helper.
+extern "C" void cfd__set_error(const char* msg);
+
+namespace {
+int vfail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  cfd__set_error(buf);
+  return code;
+}
+}  // namespace
+
+#define VCK(expr)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      rc = vfail(e_ == hipErrorOutOfMemory ? CFD_E_OOM : CFD_E_HIP, "cfd_voxelize: %s (%s)", \
+                 hipGetErrorString(e_), #expr);                                      \
+      goto done;                                                                     \
+    }                                                                                \
+  } while (0)
+
+extern "C" int cfd_voxelize(const float* pos, size_t nv, const uint32_t* idx, size_t nt,
+                            int nx, int ny, int nz, double band, int seal, uint8_t* tipo_out,
+                            float* sdf_out, cfd_vox_stats* stats) {
+  cfd__set_error("");
+  if (nx < 1 || ny < 1 || nz < 1)
+    return vfail(CFD_E_ARG, "cfd_voxelize: grid %dx%dx%d must be positive", nx, ny, nz);
+  if (!tipo_out) return vfail(CFD_E_ARG, "cfd_voxelize: tipo_out is null");
+  if ((nv && !pos) || (nt && !idx)) return vfail(CFD_E_ARG, "cfd_voxelize: null mesh arrays");
+  if (!std::isfinite(band)) return vfail(CFD_E_ARG, "cfd_voxelize: band must be finite");
+  const size_t N = (size_t)nx * ny * nz;
+  if (N >= (1ull << 31)) return vfail(CFD_E_ARG, "cfd_voxelize: grid too large");
+
+  int rc = CFD_OK;
+  hipStream_t st = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float* d_pos = nullptr;
+  uint32_t* d_idx = nullptr;
+  uint8_t *flags = nullptr, *barrier = nullptr, *ext = nullptr, *ext2 = nullptr,
+          *tipo = nullptr;
+  uint32_t* dist2 = nullptr;
+  float* sdf = nullptr;
+  unsigned long long* counters = nullptr;  // [0] casca [1] dentro [2] outside
+  int* changed = nullptr;
+  unsigned long long hc[3] = {0, 0, 0};
+  int iters = 0;
+  const unsigned cgrid = (unsigned)((N + kCellBlock - 1) / kCellBlock);
+
+  VCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  for (auto& e : ev) VCK(hipEventCreate(&e));
+  if (nv) VCK(hipMalloc(&d_pos, nv * 3 * sizeof(float)));
+  if (nt) VCK(hipMalloc(&d_idx, nt * 3 * sizeof(uint32_t)));
+  VCK(hipMalloc(&flags, N));
+  VCK(hipMalloc(&barrier, N));
+  VCK(hipMalloc(&ext, N));
+  VCK(hipMalloc(&ext2, N));
+  VCK(hipMalloc(&tipo, N));
+  VCK(hipMalloc(&dist2, N * sizeof(uint32_t)));
+  if (sdf_out) VCK(hipMalloc(&sdf, N * sizeof(float)));
+  VCK(hipMalloc(&counters, 3 * sizeof(unsigned long long)));
+  VCK(hipMalloc(&changed, sizeof(int)));
+  if (nv) VCK(hipMemcpyAsync(d_pos, pos, nv * 3 * sizeof(float), hipMemcpyHostToDevice, st));
+  if (nt) VCK(hipMemcpyAsync(d_idx, idx, nt * 3 * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  VCK(hipMemsetAsync(flags, 0, N, st));
+  VCK(hipMemsetAsync(counters, 0, 3 * sizeof(unsigned long long), st));
+  hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, st, dist2, N, INF_BITS);
+  VCK(hipGetLastError());
+
+  VCK(hipEventRecord(ev[0], st));
+  if (nt) {
+    const size_t g = nt < 262144 ? nt : 262144;
+    hipLaunchKernelGGL(k_raster, dim3((unsigned)g), dim3(kRasterBlock), 0, st, d_pos, nv, d_idx,
+                       nt, nx, ny, nz, band, flags, dist2, counters + 2);
+    VCK(hipGetLastError());
+  }
+  VCK(hipEventRecord(ev[1], st));
+
+  {
+    const uint8_t* bar = flags;
+    if (seal) {
+      hipLaunchKernelGGL(k_dilate, dim3(cgrid), dim3(kCellBlock), 0, st, flags, barrier, nx,
+                         ny, nz);
+      VCK(hipGetLastError());
+      bar = barrier;
+    }
+    hipLaunchKernelGGL(k_seed, dim3(cgrid), dim3(kCellBlock), 0, st, bar, ext, nx, ny, nz);
+    VCK(hipGetLastError());
+    const unsigned tgrid =
+        (unsigned)(((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * ((nz + TZ - 1) / TZ));
+    for (;;) {
+      VCK(hipMemsetAsync(changed, 0, sizeof(int), st));
+      hipLaunchKernelGGL(k_flood_tile, dim3(tgrid), dim3(kCellBlock), 0, st, bar, ext, nx, ny,
+                         nz, changed);
+      VCK(hipGetLastError());
+      int hchg = 0;
+      VCK(hipMemcpyAsync(&hchg, changed, sizeof(int), hipMemcpyDeviceToHost, st));
+      VCK(hipStreamSynchronize(st));
+      ++iters;
+      if (!hchg) break;
+    }
+    uint8_t* ext_final = ext;
+    if (seal) {
+      hipLaunchKernelGGL(k_skin, dim3(cgrid), dim3(kCellBlock), 0, st, ext, flags, ext2, nx, ny,
+                         nz);
+      VCK(hipGetLastError());
+      ext_final = ext2;
+    }
+    VCK(hipEventRecord(ev[2], st));
+    hipLaunchKernelGGL(k_final, dim3(cgrid), dim3(kCellBlock), 0, st, flags, ext_final, dist2,
+                       N, band, tipo, sdf, counters);
+    VCK(hipGetLastError());
+  }
+  VCK(hipEventRecord(ev[3], st));
+  VCK(hipMemcpyAsync(tipo_out, tipo, N, hipMemcpyDeviceToHost, st));
+  if (sdf_out) VCK(hipMemcpyAsync(sdf_out, sdf, N * sizeof(float), hipMemcpyDeviceToHost, st));
+  VCK(hipMemcpyAsync(hc, counters, sizeof hc, hipMemcpyDeviceToHost, st));
+  VCK(hipStreamSynchronize(st));
+  if (stats) {
+    float t01 = 0, t12 = 0, t03 = 0;
+    VCK(hipEventElapsedTime(&t01, ev[0], ev[1]));
+    VCK(hipEventElapsedTime(&t12, ev[1], ev[2]));
+    VCK(hipEventElapsedTime(&t03, ev[0], ev[3]));
+    stats->n_cells = (int64_t)N;
+    stats->n_casca = (int64_t)hc[0];
+    stats->n_dentro = (int64_t)hc[1];
+    stats->n_solido = (int64_t)(hc[0] + hc[1]);
+    stats->triangles_outside = (int64_t)hc[2];
+    stats->hollow_suspect = (hc[1] < hc[0] * 0.25 && hc[0] > 500) ? 1 : 0;
+    stats->flood_iterations = iters;
+    stats->ms_raster = t01;
+    stats->ms_flood = t12;
+    stats->ms_total = t03;
+  }
+
+done:
+  if (st) (void)hipStreamSynchronize(st);
+  for (void* p : {(void*)d_pos, (void*)d_idx, (void*)flags, (void*)barrier, (void*)ext,
+                  (void*)ext2, (void*)tipo, (void*)dist2, (void*)sdf, (void*)counters,
+                  (void*)changed})
+    if (p) (void)hipFree(p);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (st) (void)hipStreamDestroy(st);
+  return rc;
+}

@@ -1,0 +1,173 @@
+/*
+ * cfd_mi355x.h — C ABI of the MI355X D3Q19 TRT-LES solver and GPU voxelizer.
+ *
+ * Every entry point returns 0 on success and a negative code on failure; the
+ * failure text is available from cfd_last_error() (thread-local).  No C++ or
+ * torch types cross this boundary: plain pointers, sizes and PODs only.
+ *
+ * Threading: one solver handle is driven by one host thread.  All device work
+ * of a handle is issued on that handle's own HIP stream; cfd_step() and
+ * cfd_update_macros() only enqueue, every function that returns data to the
+ * host synchronizes that stream first.
+ *
+ * Reference interface each function replaces (ceschmiedel/CFD_, the
+ * SolverWebGPU object of src/backend/webgpu/solver.js and the voxelizer of
+ * src/geom/voxel.js) is cited beside each declaration.
+ */
+#ifndef CFD_MI355X_H
+#define CFD_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes (negative). */
+#define CFD_OK 0
+#define CFD_E_ARG -1     /* bad argument: size mismatch, null pointer, range   */
+#define CFD_E_HIP -2     /* HIP runtime failure (no device, launch failure)    */
+#define CFD_E_OOM -3     /* device allocation failed                           */
+#define CFD_E_STATE -4   /* call not valid in the handle's current state       */
+
+/* Storage flags for cfd_create. */
+#define CFD_STORE_FP32 0u   /* populations g_i = f_i - w_i stored as float32    */
+#define CFD_STORE_FP16 1u   /* populations stored as scaled float16 deviations */
+
+/* Cell types, identical to CELULA in src/core/emit/ir.js:71-80. */
+#define CFD_FLUIDO 0u
+#define CFD_SOLIDO 1u
+#define CFD_ENTRADA 2u
+#define CFD_SAIDA 3u
+#define CFD_SOLIDO_MOVEL 4u
+#define CFD_ESPELHO_Y 5u
+#define CFD_ESPELHO_Z 6u
+#define CFD_PAREDE 7u
+
+typedef struct cfd_solver cfd_solver;
+
+/* The 80-byte Params uniform of src/backend/webgpu/solver.js:277-293
+ * (wgsl.js:125-134).  Values are rounded to float32 on upload exactly as the
+ * reference's Float32Array.set() does. */
+typedef struct cfd_params {
+  double omega_plus;    /* even relaxation rate (molecular viscosity)          */
+  double magic;         /* TRT Lambda; 3/16 puts the wall half-way             */
+  double les_cs;        /* Smagorinsky constant; 0 disables the LES term       */
+  double belt[3];       /* SOLIDO_MOVEL wall velocity (rolling road)           */
+  double inlet[3];      /* free stream, lattice units                          */
+  double sponge_start;  /* first x of the sponge ramp                          */
+  double sponge_len;    /* sponge ramp length in cells                         */
+  double bl_thickness;  /* inlet 1/7-power boundary-layer thickness (cells)    */
+} cfd_params;
+
+typedef struct cfd_vox_stats {
+  int64_t n_cells;
+  int64_t n_casca;          /* shell voxels (VOXEL.CASCA)                      */
+  int64_t n_dentro;         /* interior voxels (VOXEL.DENTRO)                  */
+  int64_t n_solido;         /* n_casca + n_dentro                              */
+  int64_t triangles_outside;/* triangles whose padded box misses the grid     */
+  int32_t hollow_suspect;   /* ocoSuspeito: n_dentro < 0.25 n_casca, n_casca>500 */
+  int32_t flood_iterations; /* global sweeps of the exterior flood fill        */
+  double ms_raster;         /* device time of the rasterization               */
+  double ms_flood;          /* device time of seal + flood + skin recovery     */
+  double ms_total;          /* device time of the whole voxelization           */
+} cfd_vox_stats;
+
+/* SolverWebGPU.criar (solver.js:93-124) + _alocar (132-174).  Allocates two
+ * population sets (A and B), the cell-type field and the macro field, all
+ * zero-filled: g_i = 0 is fluid at rest with rho = 1 exactly (lattice.js:32-34).
+ * The front starts at A.  flags: CFD_STORE_FP32 or CFD_STORE_FP16. */
+int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver **out);
+
+/* SolverWebGPU.configurar (solver.js:277-293). */
+int cfd_configure(cfd_solver *s, const cfd_params *p);
+
+/* SolverWebGPU.definirTipos (solver.js:296-301): n must equal nx*ny*nz.
+ * Values 0..7 are CELULA codes; any other value behaves like the reference's
+ * fallthrough: streamed and collided as fluid, never a force source. */
+int cfd_set_types(cfd_solver *s, const uint32_t *tipo, size_t n);
+
+/* SolverWebGPU.inicializar (solver.js:338-349) with the init kernel of
+ * wgsl.js:269-302: equilibrium at the inlet velocity (0 in SOLIDO, beltU in
+ * SOLIDO_MOVEL) written to B; the front becomes B. */
+int cfd_init_equilibrium(cfd_solver *s);
+
+/* SolverWebGPU.definirCampo (solver.js:363-389) upload half: writes a full
+ * population field pop[i*N + cell] (i = 0..18, cell = (z*ny+y)*nx+x) into the
+ * current front.  n must equal 19*N. */
+int cfd_set_populations(cfd_solver *s, const float *soa, size_t n);
+
+/* Diagnostic read-back of the current front in the same layout (no reference
+ * counterpart; used by the parity tests).  n must equal 19*N. */
+int cfd_get_populations(cfd_solver *s, float *soa_out, size_t n);
+
+/* The writable `frente` field (solver.js:245, tests write it at
+ * nucleo.html:149): 0 = A, 1 = B. */
+int cfd_set_front(cfd_solver *s, int front);
+int cfd_get_front(cfd_solver *s, int *front);
+
+/* SolverWebGPU.passo (solver.js:395-415) device half: enqueue n steps of the
+ * fused pull-stream + TRT + Smagorinsky + boundary kernel (wgsl.js:230-259),
+ * flipping the front after each.  The macro field is written on the last of
+ * the n steps (pulled, pre-collision moments; zero in solids).  Non-blocking. */
+int cfd_step(cfd_solver *s, int n);
+
+/* Same as cfd_step, then synchronizes and returns the device time between HIP
+ * events recorded on the handle's stream around the n launches (ms). */
+int cfd_step_timed(cfd_solver *s, int n, double *ms_out);
+
+/* SolverWebGPU.medirForcas (solver.js:486-508) with shaderForcas/shaderReduzir
+ * (wgsl.js:332-435): momentum-exchange force on SOLIDO cells from the current
+ * front, accumulated in float64.  Raw value: the rest offset of
+ * calibrarRepouso (solver.js:480-484) is subtracted by the host wrapper. */
+int cfd_forces(cfd_solver *s, double out_f[3]);
+
+/* SolverWebGPU.atualizarMacros (solver.js:511-520): recompute the macro field
+ * from the current front without streaming (wgsl.js:442-476).  Non-blocking. */
+int cfd_update_macros(cfd_solver *s);
+
+/* SolverWebGPU.lerMacros (solver.js:427-442): copies the N*4 floats
+ * (ux, uy, uz, delta) to the host.  n must equal 4*N. */
+int cfd_read_macros(cfd_solver *s, float *out, size_t n);
+
+/* Float64 mass-conservation reduction (the sum of delta that the reference's
+ * nucleo.html:213-222 does on the host): sum over all non-solid cells of
+ * sum_i g_i of the current front.  Also counts non-finite populations. */
+int cfd_mass(cfd_solver *s, double *mass_out, int64_t *n_nonfinite_out);
+
+/* Blocks until all work queued on the handle's stream has finished. */
+int cfd_synchronize(cfd_solver *s);
+
+/* Device bytes held by the handle (SolverWebGPU.bytesTotais, solver.js:126-128). */
+int cfd_bytes(cfd_solver *s, int64_t *bytes_out);
+
+/* SolverWebGPU.destruir (solver.js:523-527). */
+void cfd_destroy(cfd_solver *s);
+
+/* voxelizar (src/geom/voxel.js:376-433) on the GPU: band rasterization with
+ * the exact triangle/box overlap test (61-98) and point-triangle distance
+ * (105-151) in float64 without contraction, 26-neighbour seal dilation
+ * (244-267), 6-connected exterior flood (278-327) and skin recovery (333-355).
+ * pos: nv*3 floats in lattice space; idx: nt*3 vertex indices.  tipo_out: N
+ * bytes (0 FLUIDO, 1 CASCA, 2 DENTRO); sdf_out (nullable): N floats.
+ * Bit-identical to the reference for the same inputs. */
+int cfd_voxelize(const float *pos, size_t nv, const uint32_t *idx, size_t nt,
+                 int nx, int ny, int nz, double band, int seal,
+                 uint8_t *tipo_out, float *sdf_out, cfd_vox_stats *stats);
+
+/* Achievable-HBM probe: a plain device-to-device copy kernel over `bytes`
+ * bytes, `iters` times; returns the mean copy bandwidth (read+write GB/s).
+ * Used to calibrate the roofline; no reference counterpart. */
+int cfd_copy_bandwidth(size_t bytes, int iters, double *gbps_out, double *ms_out);
+
+/* Thread-local text of the last failure ("" if none). */
+const char *cfd_last_error(void);
+
+/* Library build identification ("cfd_mi355x <version> gfx950"). */
+const char *cfd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CFD_MI355X_H */

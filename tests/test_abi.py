@@ -1,0 +1,76 @@
+"""The C-ABI library loads, exports every declared symbol and reports errors."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "cfd_mi355x.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cfd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ("cfd_create", "cfd_configure", "cfd_set_types", "cfd_init_equilibrium",
+                 "cfd_set_populations", "cfd_set_front", "cfd_step", "cfd_forces",
+                 "cfd_read_macros", "cfd_update_macros", "cfd_voxelize", "cfd_destroy",
+                 "cfd_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from cfd__amd import _lib
+    lib = C.CDLL(str(_lib.LIB_PATH))
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_lib.EXPORTS) == set(declared_functions())
+
+
+def test_version_string():
+    from cfd__amd import _lib
+    assert "gfx950" in _lib.version()
+
+
+def test_argument_errors_without_device_work():
+    """Argument validation happens before any device call."""
+    from cfd__amd import _lib
+    h = C.c_void_p()
+    rc = _lib.lib.cfd_create(0, 4, 4, 0, C.byref(h))
+    assert rc == -1 and "must be positive" in _lib.lib.cfd_last_error().decode()
+    rc = _lib.lib.cfd_create(4, 4, 4, 7, C.byref(h))
+    assert rc == -1 and "flags" in _lib.lib.cfd_last_error().decode()
+    rc = _lib.lib.cfd_create(2048, 1024, 1024, 0, C.byref(h))
+    assert rc == -1 and "2^30" in _lib.lib.cfd_last_error().decode()
+    rc = _lib.lib.cfd_step(None, 1)
+    assert rc == -1 and "null" in _lib.lib.cfd_last_error().decode()
+    rc = _lib.lib.cfd_voxelize(None, 0, None, 0, 4, 4, 4, float("nan"), 1,
+                               np.zeros(64, np.uint8).ctypes.data, None, None)
+    assert rc == -1 and "finite" in _lib.lib.cfd_last_error().decode()
+
+
+def _has_device():
+    from cfd__amd import _lib
+    h = C.c_void_p()
+    rc = _lib.lib.cfd_create(4, 4, 4, 0, C.byref(h))
+    if rc == 0:
+        _lib.lib.cfd_destroy(h)
+    return rc == 0
+
+
+def test_no_device_raises_loudly():
+    """Without a GPU the product path fails with an error; there is no CPU
+    fallback behind the ABI."""
+    from cfd__amd import _lib
+    if _has_device():
+        pytest.skip("a HIP device is visible")
+    from cfd__amd.solver import SolverMI355X
+    with pytest.raises(_lib.CFDError) as e:
+        SolverMI355X.criar({"nx": 8, "ny": 8, "nz": 8})
+    assert "device" in str(e.value).lower()
