@@ -33,6 +33,11 @@ class Params(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("block", C.c_int32), ("remap", C.c_int32), ("nontemporal", C.c_int32),
+                ("slice_pad", C.c_int32)]
+
+
 class VoxStats(C.Structure):
     _fields_ = [
         ("n_cells", C.c_int64), ("n_casca", C.c_int64), ("n_dentro", C.c_int64),
@@ -59,6 +64,8 @@ def _load() -> C.CDLL:
         "cfd_get_front": (C.c_int, [P, C.POINTER(C.c_int)]),
         "cfd_step": (C.c_int, [P, C.c_int]),
         "cfd_step_timed": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
+        "cfd_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+        "cfd_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "cfd_forces": (C.c_int, [P, C.POINTER(C.c_double)]),
         "cfd_update_macros": (C.c_int, [P]),
         "cfd_read_macros": (C.c_int, [P, P, C.c_size_t]),
@@ -86,7 +93,7 @@ lib = _load()
 EXPORTS = (
     "cfd_create", "cfd_configure", "cfd_set_types", "cfd_init_equilibrium",
     "cfd_set_populations", "cfd_get_populations", "cfd_set_front", "cfd_get_front",
-    "cfd_step", "cfd_step_timed", "cfd_forces", "cfd_update_macros", "cfd_read_macros",
+    "cfd_step", "cfd_step_timed", "cfd_set_tuning", "cfd_get_tuning", "cfd_forces", "cfd_update_macros", "cfd_read_macros",
     "cfd_mass", "cfd_synchronize", "cfd_bytes", "cfd_destroy", "cfd_voxelize",
     "cfd_copy_bandwidth", "cfd_last_error", "cfd_version",
 )

@@ -48,6 +48,7 @@ struct cfd_solver {
   Store store = Store::F32;
   size_t pop_bytes = 0;  // bytes of one population set
   Geom geom{};
+  Tuning tuning{256, 0, 0, 0};
   StepParams params{};
   bool configured = false;
   int front = 0;  // 0 = A, 1 = B
@@ -135,8 +136,8 @@ int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
   s->nz = nz;
   s->N = (size_t)N;
   s->store = flags == CFD_STORE_FP16 ? Store::F16 : Store::F32;
-  s->pop_bytes = (size_t)Q * s->N * (s->store == Store::F16 ? 2 : 4);
-  s->geom = make_geom(nx, ny, nz);
+  s->geom = make_geom(nx, ny, nz, (uint32_t)s->tuning.slice_pad);
+  s->pop_bytes = (size_t)Q * s->geom.S * (s->store == Store::F16 ? 2 : 4);
 
   auto bail = [&](int code) {
     release(s);
@@ -253,6 +254,8 @@ int cfd_init_equilibrium(cfd_solver* s) {
   return CFD_OK;
 }
 
+static size_t elem_bytes(const cfd_solver* s) { return s->store == Store::F16 ? 2 : 4; }
+
 int cfd_set_populations(cfd_solver* s, const float* soa, size_t n) {
   g_err.clear();
   if (!s || !soa) return fail(CFD_E_ARG, "cfd_set_populations: null argument");
@@ -260,15 +263,15 @@ int cfd_set_populations(cfd_solver* s, const float* soa, size_t n) {
     return fail(CFD_E_ARG, "cfd_set_populations: %zu values, expected %zu", n,
                 (size_t)Q * s->N);
   if (s->store == Store::F32) {
-    HIPCK(hipMemcpyAsync(s->pop[s->front], soa, n * sizeof(float), hipMemcpyHostToDevice,
-                         s->stream));
+    HIPCK(hipMemcpy2DAsync(s->pop[s->front], (size_t)s->geom.S * 4, soa, s->N * 4, s->N * 4, Q,
+                           hipMemcpyHostToDevice, s->stream));
     HIPCK(hipStreamSynchronize(s->stream));
     return CFD_OK;
   }
   float* tmp = nullptr;
   HIPCK(hipMalloc(&tmp, n * sizeof(float)));
   hipError_t e = hipMemcpyAsync(tmp, soa, n * sizeof(float), hipMemcpyHostToDevice, s->stream);
-  if (e == hipSuccess) e = launch_f32_to_store(s->store, tmp, s->pop[s->front], n, s->N, s->stream);
+  if (e == hipSuccess) e = launch_f32_to_store(s->store, tmp, s->pop[s->front], s->geom, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return fail(CFD_E_HIP, "cfd_set_populations: %s", hipGetErrorString(e));
@@ -282,19 +285,71 @@ int cfd_get_populations(cfd_solver* s, float* out, size_t n) {
     return fail(CFD_E_ARG, "cfd_get_populations: %zu values, expected %zu", n,
                 (size_t)Q * s->N);
   if (s->store == Store::F32) {
-    HIPCK(hipMemcpyAsync(out, s->pop[s->front], n * sizeof(float), hipMemcpyDeviceToHost,
-                         s->stream));
+    HIPCK(hipMemcpy2DAsync(out, s->N * 4, s->pop[s->front], (size_t)s->geom.S * 4, s->N * 4, Q,
+                           hipMemcpyDeviceToHost, s->stream));
     HIPCK(hipStreamSynchronize(s->stream));
     return CFD_OK;
   }
   float* tmp = nullptr;
   HIPCK(hipMalloc(&tmp, n * sizeof(float)));
-  hipError_t e = launch_store_to_f32(s->store, s->pop[s->front], tmp, n, s->N, s->stream);
+  hipError_t e = launch_store_to_f32(s->store, s->pop[s->front], tmp, s->geom, s->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(out, tmp, n * sizeof(float), hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return fail(CFD_E_HIP, "cfd_get_populations: %s", hipGetErrorString(e));
+  return CFD_OK;
+}
+
+int cfd_set_tuning(cfd_solver* s, const cfd_tuning* t) {
+  g_err.clear();
+  if (!s || !t) return fail(CFD_E_ARG, "cfd_set_tuning: null argument");
+  if (t->block != 128 && t->block != 256 && t->block != 512)
+    return fail(CFD_E_ARG, "cfd_set_tuning: block %d must be 128, 256 or 512", t->block);
+  if (t->slice_pad < 0 || t->slice_pad > (1 << 20))
+    return fail(CFD_E_ARG, "cfd_set_tuning: slice_pad %d outside [0, 2^20]", t->slice_pad);
+  if (t->slice_pad != s->tuning.slice_pad) {
+    // Re-lay both population sets with the new slice stride.
+    const Geom ng = make_geom(s->nx, s->ny, s->nz, (uint32_t)t->slice_pad);
+    const size_t eb = elem_bytes(s);
+    const size_t nbytes = (size_t)Q * ng.S * eb;
+    void* np[2] = {nullptr, nullptr};
+    for (int k = 0; k < 2; ++k) {
+      hipError_t e = hipMalloc(&np[k], nbytes);
+      if (e == hipSuccess) e = hipMemsetAsync(np[k], 0, nbytes, s->stream);
+      if (e == hipSuccess)
+        e = hipMemcpy2DAsync(np[k], (size_t)ng.S * eb, s->pop[k], (size_t)s->geom.S * eb,
+                             s->N * eb, Q, hipMemcpyDeviceToDevice, s->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+      if (e != hipSuccess) {
+        for (void* p : np)
+          if (p) (void)hipFree(p);
+        (void)hipGetLastError();
+        return fail(e == hipErrorOutOfMemory ? CFD_E_OOM : CFD_E_HIP,
+                    "cfd_set_tuning: re-layout failed: %s", hipGetErrorString(e));
+      }
+    }
+    for (int k = 0; k < 2; ++k) {
+      (void)hipFree(s->pop[k]);
+      s->pop[k] = np[k];
+    }
+    s->geom = ng;
+    s->pop_bytes = nbytes;
+  }
+  s->tuning.block = t->block;
+  s->tuning.remap = t->remap ? 1 : 0;
+  s->tuning.nontemporal = t->nontemporal ? 1 : 0;
+  s->tuning.slice_pad = t->slice_pad;
+  return CFD_OK;
+}
+
+int cfd_get_tuning(cfd_solver* s, cfd_tuning* t) {
+  g_err.clear();
+  if (!s || !t) return fail(CFD_E_ARG, "cfd_get_tuning: null argument");
+  t->block = s->tuning.block;
+  t->remap = s->tuning.remap;
+  t->nontemporal = s->tuning.nontemporal;
+  t->slice_pad = s->tuning.slice_pad;
   return CFD_OK;
 }
 
@@ -317,7 +372,7 @@ static int enqueue_steps(cfd_solver* s, int n) {
   for (int k = 0; k < n; ++k) {
     const int f = s->front;
     HIPCK(launch_step(s->store, s->pop[f], s->pop[f ^ 1], s->tipo8, s->macros, k == n - 1,
-                      s->geom, s->params, s->stream));
+                      s->geom, s->params, s->tuning, s->stream));
     s->front = f ^ 1;
   }
   return CFD_OK;

@@ -14,6 +14,7 @@ constexpr int kXcds = 8;     // MI355X: workgroups go round-robin over 8 XCDs
 // linear cell index into (x, y, z) without integer division instructions.
 struct Geom {
   uint32_t nx, ny, nz, nxny, N;
+  uint32_t S;           // elements between population slices (S >= N)
   uint32_t mnx, snx;    // n / nx   == (umulhi(n, mnx) + n) >> snx
   uint32_t mnxy, snxy;  // n / nxny == (umulhi(n, mnxy) + n) >> snxy
   uint32_t nblocks;     // logical workgroups covering N cells
@@ -35,7 +36,15 @@ struct StepParams {
   float tinf[19];   // free-stream equilibrium of the sponge  (ir.js:464-475)
 };
 
-Geom make_geom(int nx, int ny, int nz);
+Geom make_geom(int nx, int ny, int nz, uint32_t slice_pad);
+
+// Launch shape of the step kernel (chosen by the autotuner or the caller).
+struct Tuning {
+  int block;        // threads per workgroup: 128, 256 or 512
+  int remap;        // 1: contiguous slab of workgroups per XCD
+  int nontemporal;  // 1: streaming (non-temporal) population loads/stores
+  int slice_pad;    // extra elements between population slices
+};
 
 // Population storage: float32 (152 B/cell/step) or scaled float16 (76 B).
 enum class Store { F32 = 0, F16 = 1 };
@@ -44,7 +53,7 @@ hipError_t launch_prep_types(const uint32_t* tipo32, uint8_t* tipo8, const Geom&
                              hipStream_t st);
 hipError_t launch_step(Store sto, const void* src, void* dst, const uint8_t* tipo8,
                        float4* macros, bool write_macros, const Geom& g,
-                       const StepParams& p, hipStream_t st);
+                       const StepParams& p, const Tuning& tu, hipStream_t st);
 hipError_t launch_init(Store sto, void* dst, const uint8_t* tipo8, const Geom& g,
                        const StepParams& p, hipStream_t st);
 hipError_t launch_macros(Store sto, const void* src, float4* macros, const Geom& g,
@@ -56,9 +65,10 @@ hipError_t launch_forces(Store sto, const void* pop, const uint8_t* tipo8, const
                          double* partials, double* out, hipStream_t st);
 hipError_t launch_mass(Store sto, const void* pop, const uint8_t* tipo8, const Geom& g,
                        double* partials, double* out, hipStream_t st);
-hipError_t launch_f32_to_store(Store sto, const float* in, void* out, size_t n, size_t N,
+// Dense host layout [19][N] <-> device layout [19][S] (S >= N).
+hipError_t launch_f32_to_store(Store sto, const float* in, void* out, const Geom& g,
                                hipStream_t st);
-hipError_t launch_store_to_f32(Store sto, const void* in, float* out, size_t n, size_t N,
+hipError_t launch_store_to_f32(Store sto, const void* in, float* out, const Geom& g,
                                hipStream_t st);
 hipError_t launch_copy(const float4* in, float4* out, size_t n4, hipStream_t st);
 

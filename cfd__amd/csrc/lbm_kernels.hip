@@ -27,8 +27,18 @@ namespace cfd {
 
 struct StoreF32 {
   using T = float;
-  static __device__ __forceinline__ float ld(const T* p) { return *p; }
-  static __device__ __forceinline__ void st(T* p, float v) { *p = v; }
+  template <bool NT = false>
+  static __device__ __forceinline__ float ld(const T* p) {
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
+  }
+  template <bool NT = false>
+  static __device__ __forceinline__ void st(T* p, float v) {
+    if (NT)
+      __builtin_nontemporal_store(v, p);
+    else
+      *p = v;
+  }
 };
 
 // float16 deviations g_i = f_i - w_i scaled by 2^12 (exact power of two, so
@@ -36,15 +46,37 @@ struct StoreF32 {
 // down to 1.5e-8 stay normal.  Collision still runs in float32.
 struct StoreF16 {
   using T = _Float16;
+  template <bool NT = false>
   static __device__ __forceinline__ float ld(const T* p) {
-    return static_cast<float>(*p) * 0.000244140625f;
+    const T h = NT ? __builtin_nontemporal_load(p) : *p;
+    return static_cast<float>(h) * 0.000244140625f;
   }
+  template <bool NT = false>
   static __device__ __forceinline__ void st(T* p, float v) {
-    *p = static_cast<T>(v * 4096.0f);
+    const T h = static_cast<T>(v * 4096.0f);
+    if (NT)
+      __builtin_nontemporal_store(h, p);
+    else
+      *p = h;
   }
 };
 
 // ------------------------------------------------------------------ indexing
+
+// Element `idx` of the slice starting at `base`: a wave-uniform 64-bit base
+// plus a 32-bit byte offset, so the load uses the scalar-base addressing form
+// (no per-lane 64-bit address arithmetic).  idx * sizeof(T) < 2^32 because a
+// handle holds fewer than 2^30 cells.
+template <class T>
+__device__ __forceinline__ const T* at(const T* base, uint32_t idx) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                    (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ T* at(T* base, uint32_t idx) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) +
+                              (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
 
 __device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, uint32_t s) {
   return (__umulhi(n, m) + n) >> s;
@@ -55,6 +87,10 @@ __device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, uint32_t s)
 __device__ __forceinline__ uint32_t logical_block(const Geom& g) {
   const uint32_t b = blockIdx.x;
   return (b % kXcds) * g.per_xcd + b / kXcds;
+}
+__device__ __forceinline__ uint32_t logical_block(uint32_t per_xcd, int remap) {
+  const uint32_t b = blockIdx.x;
+  return remap ? (b % kXcds) * per_xcd + b / kXcds : b;
 }
 
 struct Coord {
@@ -149,16 +185,35 @@ __global__ __launch_bounds__(kBlock) void k_prep_types(const uint32_t* __restric
 // pull streaming with halfway bounce-back at the fluid node, moments, then
 // the branch chain ESPELHO_Y -> ESPELHO_Z -> ENTRADA -> SAIDA -> fluid
 // (equilibrium, closed-form Smagorinsky, TRT per opposite pair, sponge).
-template <class S, bool MACROS>
-__global__ __launch_bounds__(kBlock) void k_step(const typename S::T* __restrict__ src,
-                                                 typename S::T* __restrict__ dst,
-                                                 const uint8_t* __restrict__ tipo,
-                                                 float4* __restrict__ macros, Geom g,
-                                                 StepParams p) {
-  using T = typename S::T;
-  const uint32_t cell = logical_block(g) * kBlock + threadIdx.x;
+template <class S, bool MACROS, int BLOCK, bool NT>
+__global__ __launch_bounds__(BLOCK) void k_step(const typename S::T* __restrict__ src,
+                                                typename S::T* __restrict__ dst,
+                                                const uint8_t* __restrict__ tipo,
+                                                float4* __restrict__ macros, Geom g,
+                                                StepParams p, uint32_t per_xcd, int remap) {
+  const uint32_t cell = logical_block(per_xcd, remap) * BLOCK + threadIdx.x;
   if (cell >= g.N) return;
+  const Coord c = split(cell, g);
+  const Wrap w = wrap_offsets(c, g);
+  const size_t N = g.S;  // slice stride
+
+  // 1. pull (ir.js:166-189): g_i = src[i][x - c_i], or, when x - c_i is a
+  //    wall, the population this cell sent the other way last step plus the
+  //    moving-wall term 2 w_i (c_i . u_w) / c_s^2 with u_w = beltU on
+  //    SOLIDO_MOVEL and 0 elsewhere.
+  //    The pull addresses depend only on the coordinates, so the cell's type
+  //    byte and all 19 streamed values are requested together: a cell away
+  //    from walls costs one memory round trip.  Only cells flagged NEAR_WALL
+  //    look at their neighbours' types and re-load the bounced directions.
   const uint32_t tb = tipo[cell];
+  float gi[Q];
+  uint32_t sidx[Q];
+  gi[0] = S::template ld<NT>(at(src, cell));
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    sidx[i] = (uint32_t)((int)cell + nb_offset(w, -cx(i), -cy(i), -cz(i)));
+    gi[i] = S::template ld<NT>(at(src + (size_t)i * N, sidx[i]));
+  }
   const uint32_t ct = tb & TYPE_MASK;
 
   // Walls are not simulated: nothing ever reads their populations
@@ -167,35 +222,15 @@ __global__ __launch_bounds__(kBlock) void k_step(const typename S::T* __restrict
     if (MACROS) macros[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
   }
-
-  const Coord c = split(cell, g);
-  const Wrap w = wrap_offsets(c, g);
-  const size_t N = g.N;
-
-  // 1. pull (ir.js:166-189): g_i = src[i][x - c_i], or, when x - c_i is a
-  //    wall, the population this cell sent the other way last step plus the
-  //    moving-wall term 2 w_i (c_i . u_w) / c_s^2 with u_w = beltU on
-  //    SOLIDO_MOVEL and 0 elsewhere.
-  float gi[Q];
-  gi[0] = S::ld(src + cell);
-  if (!(tb & NEAR_WALL)) {
+  if (tb & NEAR_WALL) {
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
-      const uint32_t s = (uint32_t)((int)cell + nb_offset(w, -cx(i), -cy(i), -cz(i)));
-      gi[i] = S::ld(src + (size_t)i * N + s);
-    }
-  } else {
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      const uint32_t s = (uint32_t)((int)cell + nb_offset(w, -cx(i), -cy(i), -cz(i)));
-      const uint32_t t = tipo[s] & TYPE_MASK;
+      const uint32_t t = tipo[sidx[i]] & TYPE_MASK;
       if (is_wall(t)) {
         const bool moving = (t == SOLIDO_MOVEL);
         const float cu = dotc(i, moving ? p.belt[0] : 0.f, moving ? p.belt[1] : 0.f,
                               moving ? p.belt[2] : 0.f);
-        gi[i] = S::ld(src + (size_t)opp(i) * N + cell) + kwall(i) * cu;
-      } else {
-        gi[i] = S::ld(src + (size_t)i * N + s);
+        gi[i] = S::template ld<NT>(at(src + (size_t)opp(i) * N, cell)) + kwall(i) * cu;
       }
     }
   }
@@ -316,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_step(const typename S::T* __restrict
   }
 
 #pragma unroll
-  for (int i = 0; i < Q; ++i) S::st(dst + (size_t)i * N + cell, d[i]);
+  for (int i = 0; i < Q; ++i) S::template st<NT>(at(dst + (size_t)i * N, cell), d[i]);
   if (MACROS) macros[cell] = make_float4(ux, uy, uz, delta);
 }
 
@@ -339,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_init(typename S::T* __restrict__ dst
     uz = p.belt[2];
   }
   const float uu = ux * ux + uy * uy + uz * uz;
-  const size_t N = g.N;
+  const size_t N = g.S;  // slice stride
   S::st(dst + cell, wf(0) * (0.0f - 1.5f * 1.0f * uu));
 #pragma unroll
   for (int i = 1; i < Q; ++i) {
@@ -357,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_macros(const typename S::T* __restri
                                                    float4* __restrict__ macros, Geom g) {
   const uint32_t cell = logical_block(g) * kBlock + threadIdx.x;
   if (cell >= g.N) return;
-  const size_t N = g.N;
+  const size_t N = g.S;  // slice stride
   float gi[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) gi[i] = S::ld(src + (size_t)i * N + cell);
@@ -422,7 +457,7 @@ template <class S>
 __global__ __launch_bounds__(kBlock) void k_forces(const typename S::T* __restrict__ pop,
                                                    const uint8_t* __restrict__ tipo, Geom g,
                                                    double* __restrict__ partials) {
-  const size_t N = g.N;
+  const size_t N = g.S;  // slice stride
   double f[3] = {0.0, 0.0, 0.0};
   for (uint32_t cell = blockIdx.x * kBlock + threadIdx.x; cell < g.N;
        cell += gridDim.x * kBlock) {
@@ -448,7 +483,7 @@ template <class S>
 __global__ __launch_bounds__(kBlock) void k_mass(const typename S::T* __restrict__ pop,
                                                  const uint8_t* __restrict__ tipo, Geom g,
                                                  double* __restrict__ partials) {
-  const size_t N = g.N;
+  const size_t N = g.S;  // slice stride
   double v[2] = {0.0, 0.0};
   for (uint32_t cell = blockIdx.x * kBlock + threadIdx.x; cell < g.N;
        cell += gridDim.x * kBlock) {
@@ -488,24 +523,39 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const double* __restrict__ pa
 
 template <class S>
 __global__ void k_from_f32(const float* __restrict__ in, typename S::T* __restrict__ out,
-                           size_t n) {
+                           Geom g) {
+  const size_t n = (size_t)Q * g.N;
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
        k += (size_t)gridDim.x * blockDim.x)
-    S::st(out + k, in[k]);
+    S::st(out + (k / g.N) * g.S + k % g.N, in[k]);
 }
 template <class S>
 __global__ void k_to_f32(const typename S::T* __restrict__ in, float* __restrict__ out,
-                         size_t n) {
+                         Geom g) {
+  const size_t n = (size_t)Q * g.N;
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
        k += (size_t)gridDim.x * blockDim.x)
-    out[k] = S::ld(in + k);
+    out[k] = S::ld(in + (k / g.N) * g.S + k % g.N);
 }
 
-__global__ __launch_bounds__(kBlock) void k_copy(const float4* __restrict__ in,
-                                                 float4* __restrict__ out, size_t n4) {
-  for (size_t k = blockIdx.x * (size_t)kBlock + threadIdx.x; k < n4;
-       k += (size_t)gridDim.x * kBlock)
-    out[k] = in[k];
+// Bandwidth probe: each thread moves 4 float4 (64 B), workgroup-strided so
+// every load instruction of a wave covers 1 KiB contiguously.
+typedef float vf4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(kBlock) void k_copy(const vf4* __restrict__ in,
+                                                 vf4* __restrict__ out, size_t n4) {
+  const size_t base = (size_t)blockIdx.x * kBlock * 4 + threadIdx.x;
+  vf4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const size_t j = base + (size_t)k * kBlock;
+    if (j < n4) v[k] = __builtin_nontemporal_load(in + j);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const size_t j = base + (size_t)k * kBlock;
+    if (j < n4) __builtin_nontemporal_store(v[k], out + j);
+  }
 }
 
 // ------------------------------------------------------------------ host side
@@ -521,13 +571,15 @@ static void magic_u32(uint32_t d, uint32_t* m, uint32_t* s) {
   *s = l;
 }
 
-Geom make_geom(int nx, int ny, int nz) {
+Geom make_geom(int nx, int ny, int nz, uint32_t slice_pad) {
   Geom g;
   g.nx = (uint32_t)nx;
   g.ny = (uint32_t)ny;
   g.nz = (uint32_t)nz;
   g.nxny = g.nx * g.ny;
   g.N = g.nxny * g.nz;
+  // slices start on 256-element boundaries, plus the staggering pad
+  g.S = ((g.N + 255u) & ~255u) + slice_pad;
   magic_u32(g.nx, &g.mnx, &g.snx);
   magic_u32(g.nxny, &g.mnxy, &g.snxy);
   g.nblocks = (g.N + kBlock - 1) / kBlock;
@@ -542,26 +594,47 @@ hipError_t launch_prep_types(const uint32_t* t32, uint8_t* t8, const Geom& g, hi
   return hipGetLastError();
 }
 
+template <class S, bool M>
+static void step_dispatch(const void* src, void* dst, const uint8_t* tipo8, float4* macros,
+                          const Geom& g, const StepParams& p, const Tuning& tu,
+                          hipStream_t st) {
+  auto s = static_cast<const typename S::T*>(src);
+  auto d = static_cast<typename S::T*>(dst);
+  const uint32_t nb = (g.N + tu.block - 1) / tu.block;
+  const uint32_t per = tu.remap ? (nb + kXcds - 1) / kXcds : nb;
+  const dim3 grid(tu.remap ? per * kXcds : nb);
+#define CFD_STEP_CASE(B, NTV)                                                             \
+  hipLaunchKernelGGL((k_step<S, M, B, NTV>), grid, dim3(B), 0, st, s, d, tipo8, macros, g, p, \
+                     per, tu.remap)
+  const bool nt = tu.nontemporal != 0;
+  switch (tu.block) {
+    case 128:
+      if (nt) CFD_STEP_CASE(128, true); else CFD_STEP_CASE(128, false);
+      break;
+    case 512:
+      if (nt) CFD_STEP_CASE(512, true); else CFD_STEP_CASE(512, false);
+      break;
+    default:
+      if (nt) CFD_STEP_CASE(256, true); else CFD_STEP_CASE(256, false);
+      break;
+  }
+#undef CFD_STEP_CASE
+}
+
 hipError_t launch_step(Store sto, const void* src, void* dst, const uint8_t* tipo8,
                        float4* macros, bool write_macros, const Geom& g,
-                       const StepParams& p, hipStream_t st) {
-  const dim3 grid = cell_grid(g), blk(kBlock);
+                       const StepParams& p, const Tuning& tu, hipStream_t st) {
+  if (tu.block != 128 && tu.block != 256 && tu.block != 512) return hipErrorInvalidValue;
   if (sto == Store::F32) {
-    using S = StoreF32;
-    auto s = static_cast<const S::T*>(src);
-    auto d = static_cast<S::T*>(dst);
     if (write_macros)
-      hipLaunchKernelGGL((k_step<S, true>), grid, blk, 0, st, s, d, tipo8, macros, g, p);
+      step_dispatch<StoreF32, true>(src, dst, tipo8, macros, g, p, tu, st);
     else
-      hipLaunchKernelGGL((k_step<S, false>), grid, blk, 0, st, s, d, tipo8, macros, g, p);
+      step_dispatch<StoreF32, false>(src, dst, tipo8, macros, g, p, tu, st);
   } else {
-    using S = StoreF16;
-    auto s = static_cast<const S::T*>(src);
-    auto d = static_cast<S::T*>(dst);
     if (write_macros)
-      hipLaunchKernelGGL((k_step<S, true>), grid, blk, 0, st, s, d, tipo8, macros, g, p);
+      step_dispatch<StoreF16, true>(src, dst, tipo8, macros, g, p, tu, st);
     else
-      hipLaunchKernelGGL((k_step<S, false>), grid, blk, 0, st, s, d, tipo8, macros, g, p);
+      step_dispatch<StoreF16, false>(src, dst, tipo8, macros, g, p, tu, st);
   }
   return hipGetLastError();
 }
@@ -626,32 +699,34 @@ static dim3 flat_grid(size_t n) {
   return dim3((unsigned)b);
 }
 
-hipError_t launch_f32_to_store(Store sto, const float* in, void* out, size_t n, size_t,
+hipError_t launch_f32_to_store(Store sto, const float* in, void* out, const Geom& g,
                                hipStream_t st) {
+  const size_t n = (size_t)Q * g.N;
   if (sto == Store::F32)
     hipLaunchKernelGGL(k_from_f32<StoreF32>, flat_grid(n), dim3(kBlock), 0, st, in,
-                       static_cast<float*>(out), n);
+                       static_cast<float*>(out), g);
   else
     hipLaunchKernelGGL(k_from_f32<StoreF16>, flat_grid(n), dim3(kBlock), 0, st, in,
-                       static_cast<_Float16*>(out), n);
+                       static_cast<_Float16*>(out), g);
   return hipGetLastError();
 }
 
-hipError_t launch_store_to_f32(Store sto, const void* in, float* out, size_t n, size_t,
+hipError_t launch_store_to_f32(Store sto, const void* in, float* out, const Geom& g,
                                hipStream_t st) {
+  const size_t n = (size_t)Q * g.N;
   if (sto == Store::F32)
     hipLaunchKernelGGL(k_to_f32<StoreF32>, flat_grid(n), dim3(kBlock), 0, st,
-                       static_cast<const float*>(in), out, n);
+                       static_cast<const float*>(in), out, g);
   else
     hipLaunchKernelGGL(k_to_f32<StoreF16>, flat_grid(n), dim3(kBlock), 0, st,
-                       static_cast<const _Float16*>(in), out, n);
+                       static_cast<const _Float16*>(in), out, g);
   return hipGetLastError();
 }
 
 hipError_t launch_copy(const float4* in, float4* out, size_t n4, hipStream_t st) {
-  size_t b = (n4 + kBlock - 1) / kBlock;
-  if (b > 8192) b = 8192;
-  hipLaunchKernelGGL(k_copy, dim3((unsigned)b), dim3(kBlock), 0, st, in, out, n4);
+  const size_t b = (n4 + kBlock * 4 - 1) / (kBlock * 4);
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)b), dim3(kBlock), 0, st,
+                     reinterpret_cast<const vf4*>(in), reinterpret_cast<vf4*>(out), n4);
   return hipGetLastError();
 }
 

@@ -182,6 +182,22 @@ class SolverMI355X:
         self.passos += n
         return ms.value
 
+    @property
+    def ajuste(self) -> dict:
+        """Launch tuning of the step kernel (block, remap, nontemporal, slice_pad)."""
+        t = _lib.Tuning()
+        check(lib.cfd_get_tuning(self._h, C.byref(t)))
+        return {"block": t.block, "remap": t.remap, "nontemporal": t.nontemporal,
+                "slice_pad": t.slice_pad}
+
+    @ajuste.setter
+    def ajuste(self, v: dict) -> None:
+        cur = self.ajuste
+        cur.update(v)
+        t = _lib.Tuning(int(cur["block"]), int(cur["remap"]), int(cur["nontemporal"]),
+                        int(cur["slice_pad"]))
+        check(lib.cfd_set_tuning(self._h, C.byref(t)))
+
     def sincronizar(self) -> None:
         check(lib.cfd_synchronize(self._h))
 

@@ -61,6 +61,15 @@ typedef struct cfd_params {
   double bl_thickness;  /* inlet 1/7-power boundary-layer thickness (cells)    */
 } cfd_params;
 
+/* Launch shape and layout of the step kernel.  No reference counterpart:
+ * the WGSL path has one fixed 64x1x1 workgroup (wgsl.js:186). */
+typedef struct cfd_tuning {
+  int32_t block;        /* threads per workgroup: 128, 256 or 512              */
+  int32_t remap;        /* 1: each XCD streams one contiguous slab of cells    */
+  int32_t nontemporal;  /* 1: non-temporal population loads and stores         */
+  int32_t slice_pad;    /* extra elements between the 19 population slices     */
+} cfd_tuning;
+
 typedef struct cfd_vox_stats {
   int64_t n_cells;
   int64_t n_casca;          /* shell voxels (VOXEL.CASCA)                      */
@@ -112,6 +121,12 @@ int cfd_get_front(cfd_solver *s, int *front);
  * flipping the front after each.  The macro field is written on the last of
  * the n steps (pulled, pre-collision moments; zero in solids).  Non-blocking. */
 int cfd_step(cfd_solver *s, int n);
+
+/* Launch tuning of the step kernel (autotuner hook; no reference
+ * counterpart).  Changing slice_pad re-lays both population sets in place
+ * (contents preserved).  Results do not depend on the tuning. */
+int cfd_set_tuning(cfd_solver *s, const cfd_tuning *t);
+int cfd_get_tuning(cfd_solver *s, cfd_tuning *t);
 
 /* Same as cfd_step, then synchronizes and returns the device time between HIP
  * events recorded on the handle's stream around the n launches (ms). */
