@@ -1,0 +1,158 @@
+"""Voxelizer: oracle and GPU against the reference's own voxelizar() outputs.
+
+tests/golden/voxel_golden.json holds, per case, sha256 of the tipo and sdf
+arrays that the reference's src/geom/voxel.js produced (generated by
+tests/golden/make_voxel_golden.py in the development container), plus the
+full tipo as run-length pairs for the small cases.  The bar is bit-exact.
+"""
+import hashlib
+import json
+import math
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE / "golden"))
+
+from voxel_cases import CUBE_STL_TRIANGLES, cases  # noqa: E402
+
+from cfd__amd import meshes  # noqa: E402
+
+GOLDEN = json.loads((HERE / "golden" / "voxel_golden.json").read_text())["cases"]
+CASES = cases()
+SMALL = [n for n in CASES if GOLDEN[n]["grid"][0] * GOLDEN[n]["grid"][1] * GOLDEN[n]["grid"][2]
+         < 200_000]
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _unrle(pairs):
+    return np.concatenate([np.full(n, v, dtype=np.uint8) for v, n in pairs])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_mesh_generators_are_deterministic(name):
+    pos, idx, *_ = CASES[name]()
+    assert meshes.mesh_digest(pos, idx) == GOLDEN[name]["mesh_sha256"]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_oracle_matches_reference_voxelizer(orc, name):
+    pos, idx, grid, banda, selar, _ = CASES[name]()
+    tipo, sdf, st = orc.voxelize(pos, idx, grid, banda, selar)
+    g = GOLDEN[name]
+    assert _sha(tipo) == g["tipo_sha256"]
+    assert _sha(sdf) == g["sdf_sha256"]
+    assert (int(st[0]), int(st[1]), int(st[3]), bool(st[4])) == \
+        (g["nCasca"], g["nDentro"], g["trianguloFora"], g["ocoSuspeito"])
+    if "tipo_rle" in g:
+        assert np.array_equal(_unrle(g["tipo_rle"]), tipo)
+
+
+def _cube_expected(scale, offset, grid):
+    """Closed form for an axis-aligned cube [a, b]^3: a voxel is shell when its
+    closed box meets the cube surface, interior when it lies strictly
+    inside, fluid otherwise (conservative overlap test + flood + skin)."""
+    nx, ny, nz = grid
+    lo = np.asarray(offset, float)
+    hi = lo + scale
+    out = np.zeros((nz, ny, nx), dtype=np.uint8)
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                p = np.array([x, y, z], float)
+                meets = np.all(p <= hi) and np.all(p + 1 >= lo)
+                inside = np.all(p > lo) and np.all(p + 1 < hi)
+                out[z, y, x] = 2 if inside else (1 if meets else 0)
+    return out.reshape(-1)
+
+
+@pytest.mark.parametrize("name,scale,offset", [
+    ("cube_stl_fractional", 7.3, (3.4, 2.6, 4.15)),
+    ("cube_stl_integer_aligned", 8.0, (4.0, 4.0, 4.0)),
+])
+def test_reference_cube_matches_closed_form(name, scale, offset):
+    """The golden vectors themselves agree with the hand-derivable answer for
+    the reference's cube_example.stl scaled into the grid."""
+    g = GOLDEN[name]
+    assert np.array_equal(_unrle(g["tipo_rle"]), _cube_expected(scale, offset, g["grid"]))
+
+
+def test_cube_fixture_matches_reference_file():
+    """CUBE_STL_TRIANGLES are the triangles of examples/cube_example.stl
+    (79-byte header: count at offset 79, 79 + 4 + 12*50 = 683 bytes)."""
+    f = Path("/root/reference/examples/cube_example.stl")
+    if not f.exists():
+        pytest.skip("reference checkout not present (GPU box)")
+    from cfd__amd.mesh_io import parseSTL
+    m = parseSTL(f.read_bytes())
+    assert m["formato"] == "stl-binario" and m["deslocado"] == -1
+    assert np.array_equal(m["positions"].reshape(-1, 9), CUBE_STL_TRIANGLES.astype(np.float32))
+
+
+def test_sphere_solid_set_bounds(orc):
+    """Conservative classification of a closed icosphere (radius r, centre c):
+    every cell whose box lies inside the inscribed ball is solid, and no
+    solid cell centre is farther than r + sqrt(3)/2 + sqrt(3) from c: a shell
+    voxel touches the surface, and the seal dilation can leave a cell of the
+    26-neighbour band unrecovered (no face neighbour in the flooded
+    exterior).  The frontal shadow lies between the two discs."""
+    pos, idx, grid, banda, selar, _ = CASES["sphere_20k_160x80x64"]()
+    tipo, _, st = orc.voxelize(pos, idx, grid, banda, selar)
+    nx, ny, nz = grid
+    c = np.array([48.0, 40.0, 32.0])
+    v = pos.reshape(-1, 3).astype(np.float64) - c
+    r_out = np.sqrt((v * v).sum(axis=1)).max()
+    # inradius of the refined icosahedron: distance of the face planes
+    t = v[idx.reshape(-1, 3)]
+    n = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+    r_in = np.abs((n * t[:, 0]).sum(axis=1) / np.linalg.norm(n, axis=1)).min()
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    solid = (tipo.reshape(nz, ny, nx) != 0)
+    far = np.zeros_like(solid)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                d2 = (x + dx - c[0]) ** 2 + (y + dy - c[1]) ** 2 + (z + dz - c[2]) ** 2
+                far |= d2 >= r_in ** 2
+    inside = ~far
+    assert solid[inside].all()
+    dc = np.sqrt((x + 0.5 - c[0]) ** 2 + (y + 0.5 - c[1]) ** 2 + (z + 0.5 - c[2]) ** 2)
+    assert dc[solid].max() <= r_out + 1.5 * math.sqrt(3) + 1e-9
+    from cfd__amd.tunel import areaFrontal
+    a = areaFrontal(tipo, {"nx": nx, "ny": ny, "nz": nz})
+    assert math.pi * (r_in - 1.5) ** 2 <= a <= math.pi * (r_out + 1.5 * math.sqrt(3)) ** 2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(CASES))
+def test_gpu_voxelizer_bit_exact(name):
+    from cfd__amd.voxel import voxelizar
+    pos, idx, grid, banda, selar, _ = CASES[name]()
+    nx, ny, nz = grid
+    r = voxelizar(pos, idx, {"nx": nx, "ny": ny, "nz": nz}, banda=banda, selarFrestas=selar)
+    g = GOLDEN[name]
+    assert _sha(r["tipo"]) == g["tipo_sha256"], name
+    assert _sha(r["sdf"]) == g["sdf_sha256"], name
+    s = r["stats"]
+    assert (s["nCasca"], s["nDentro"], s["trianguloFora"], s["ocoSuspeito"]) == \
+        (g["nCasca"], g["nDentro"], g["trianguloFora"], g["ocoSuspeito"])
+
+
+@pytest.mark.gpu
+def test_gpu_voxelizer_repeatable_and_empty_mesh():
+    """Two runs give the same bytes (order-free atomics), and an empty mesh is
+    all fluid."""
+    from cfd__amd.voxel import voxelizar
+    pos, idx, grid, banda, selar, _ = CASES["random_soup_300"]()
+    g = {"nx": grid[0], "ny": grid[1], "nz": grid[2]}
+    a = voxelizar(pos, idx, g, banda, selar)
+    b = voxelizar(pos, idx, g, banda, selar)
+    assert np.array_equal(a["tipo"], b["tipo"]) and np.array_equal(a["sdf"], b["sdf"])
+    e = voxelizar(np.zeros(0, np.float32), np.zeros(0, np.uint32), g)
+    assert (e["tipo"] == 0).all() and (e["sdf"] == 2.5).all()
