@@ -88,8 +88,10 @@ def run_gpu(case, precision: str, steps: int, warmup: int):
 
     g, units = case["grid"], case["units"]
     s = SolverMI355X.criar(g, precisao=precision)
+    # fixed floor: the app feeds a 1/7-power inlet boundary layer of
+    # round(0.12 nz) cells (index.html:727-734)
     cfg = dict(omegaPlus=units.omegaPlus, lesCs=0.1, esponja=case["esponja"],
-               beltU=[0.0, 0.0, 0.0])
+               beltU=[0.0, 0.0, 0.0], espessuraCL=float(math.floor(g["nz"] * 0.12 + 0.5)))
 
     def aplicar(f):
         s.configurar(inletU=[U_IN * f, 0.0, 0.0], **cfg)
@@ -138,7 +140,8 @@ def cpu_baseline(case, budget_s: float):
 
     nx, ny, nz = GRID
     par = oracle.params_array(omegaPlus=case["units"].omegaPlus, lesCs=0.1,
-                              inletU=[U_IN, 0.0, 0.0], esponja=case["esponja"], nx=nx)
+                              inletU=[U_IN, 0.0, 0.0], esponja=case["esponja"],
+                              espessuraCL=float(math.floor(nz * 0.12 + 0.5)), nx=nx)
     a = oracle.init(GRID, case["tipo"], par)
     b = a.copy()
     lib = oracle.lib()
@@ -203,6 +206,7 @@ def main():
             "omega_plus_used": u.omegaPlus, "re_lattice_used": u.reLattice,
             "omega_plus_requested": 1 / (3 * U_IN * BODY_LENGTH / RE_REQUESTED + 0.5),
             "re_requested": RE_REQUESTED, "floor": "PAREDE (no-slip)",
+            "inlet_boundary_layer_cells": math.floor(nz * 0.12 + 0.5),
             "parallelism": "single GPU",
         },
         "hbm_frac_of_peak": round(achieved / HBM_PEAK_GBS, 4),

@@ -34,7 +34,7 @@ class Params(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("block", C.c_int32), ("remap", C.c_int32), ("nontemporal", C.c_int32),
+    _fields_ = [("block", C.c_int32), ("remap", C.c_int32), ("layout", C.c_int32),
                 ("slice_pad", C.c_int32)]
 
 

@@ -48,7 +48,7 @@ struct cfd_solver {
   Store store = Store::F32;
   size_t pop_bytes = 0;  // bytes of one population set
   Geom geom{};
-  Tuning tuning{256, 0, 0, 0};
+  Tuning tuning{512, 0, kLayoutBlocked, 0};
   StepParams params{};
   bool configured = false;
   int front = 0;  // 0 = A, 1 = B
@@ -102,6 +102,8 @@ float dotc_host(int i, const float v[3]) {
   return acc;
 }
 
+size_t elem_bytes(const cfd_solver* s) { return s->store == Store::F16 ? 2 : 4; }
+
 }  // namespace
 
 extern "C" {
@@ -136,8 +138,14 @@ int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
   s->nz = nz;
   s->N = (size_t)N;
   s->store = flags == CFD_STORE_FP16 ? Store::F16 : Store::F32;
-  s->geom = make_geom(nx, ny, nz, (uint32_t)s->tuning.slice_pad);
-  s->pop_bytes = (size_t)Q * s->geom.S * (s->store == Store::F16 ? 2 : 4);
+  const size_t eb = s->store == Store::F16 ? 2 : 4;
+  s->geom = make_geom(nx, ny, nz, s->tuning.layout, (uint32_t)s->tuning.slice_pad);
+  if (s->geom.pop_elems * eb >= (1ull << 32)) {
+    // the blocked layout addresses a set with 32-bit byte offsets
+    s->tuning.layout = kLayoutSoA;
+    s->geom = make_geom(nx, ny, nz, kLayoutSoA, (uint32_t)s->tuning.slice_pad);
+  }
+  s->pop_bytes = s->geom.pop_elems * eb;
 
   auto bail = [&](int code) {
     release(s);
@@ -254,7 +262,31 @@ int cfd_init_equilibrium(cfd_solver* s) {
   return CFD_OK;
 }
 
-static size_t elem_bytes(const cfd_solver* s) { return s->store == Store::F16 ? 2 : 4; }
+// Dense [19][N] float32 on the device <-> the handle's storage layout.
+static int upload_dense(cfd_solver* s, const float* host, void* dst) {
+  const size_t n = (size_t)Q * s->N;
+  float* tmp = nullptr;
+  HIPCK(hipMalloc(&tmp, n * sizeof(float)));
+  hipError_t e = hipMemcpyAsync(tmp, host, n * sizeof(float), hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = launch_f32_to_store(s->store, tmp, dst, s->geom, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(CFD_E_HIP, "population upload: %s", hipGetErrorString(e));
+  return CFD_OK;
+}
+
+static int download_dense(cfd_solver* s, const void* src, float* host) {
+  const size_t n = (size_t)Q * s->N;
+  float* tmp = nullptr;
+  HIPCK(hipMalloc(&tmp, n * sizeof(float)));
+  hipError_t e = launch_store_to_f32(s->store, src, tmp, s->geom, s->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(host, tmp, n * sizeof(float), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(CFD_E_HIP, "population read-back: %s", hipGetErrorString(e));
+  return CFD_OK;
+}
 
 int cfd_set_populations(cfd_solver* s, const float* soa, size_t n) {
   g_err.clear();
@@ -262,20 +294,7 @@ int cfd_set_populations(cfd_solver* s, const float* soa, size_t n) {
   if (n != (size_t)Q * s->N)
     return fail(CFD_E_ARG, "cfd_set_populations: %zu values, expected %zu", n,
                 (size_t)Q * s->N);
-  if (s->store == Store::F32) {
-    HIPCK(hipMemcpy2DAsync(s->pop[s->front], (size_t)s->geom.S * 4, soa, s->N * 4, s->N * 4, Q,
-                           hipMemcpyHostToDevice, s->stream));
-    HIPCK(hipStreamSynchronize(s->stream));
-    return CFD_OK;
-  }
-  float* tmp = nullptr;
-  HIPCK(hipMalloc(&tmp, n * sizeof(float)));
-  hipError_t e = hipMemcpyAsync(tmp, soa, n * sizeof(float), hipMemcpyHostToDevice, s->stream);
-  if (e == hipSuccess) e = launch_f32_to_store(s->store, tmp, s->pop[s->front], s->geom, s->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  (void)hipFree(tmp);
-  if (e != hipSuccess) return fail(CFD_E_HIP, "cfd_set_populations: %s", hipGetErrorString(e));
-  return CFD_OK;
+  return upload_dense(s, soa, s->pop[s->front]);
 }
 
 int cfd_get_populations(cfd_solver* s, float* out, size_t n) {
@@ -284,50 +303,46 @@ int cfd_get_populations(cfd_solver* s, float* out, size_t n) {
   if (n != (size_t)Q * s->N)
     return fail(CFD_E_ARG, "cfd_get_populations: %zu values, expected %zu", n,
                 (size_t)Q * s->N);
-  if (s->store == Store::F32) {
-    HIPCK(hipMemcpy2DAsync(out, s->N * 4, s->pop[s->front], (size_t)s->geom.S * 4, s->N * 4, Q,
-                           hipMemcpyDeviceToHost, s->stream));
-    HIPCK(hipStreamSynchronize(s->stream));
-    return CFD_OK;
-  }
-  float* tmp = nullptr;
-  HIPCK(hipMalloc(&tmp, n * sizeof(float)));
-  hipError_t e = launch_store_to_f32(s->store, s->pop[s->front], tmp, s->geom, s->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(out, tmp, n * sizeof(float), hipMemcpyDeviceToHost, s->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  (void)hipFree(tmp);
-  if (e != hipSuccess) return fail(CFD_E_HIP, "cfd_get_populations: %s", hipGetErrorString(e));
-  return CFD_OK;
+  return download_dense(s, s->pop[s->front], out);
 }
 
 int cfd_set_tuning(cfd_solver* s, const cfd_tuning* t) {
   g_err.clear();
   if (!s || !t) return fail(CFD_E_ARG, "cfd_set_tuning: null argument");
-  if (t->block != 128 && t->block != 256 && t->block != 512)
-    return fail(CFD_E_ARG, "cfd_set_tuning: block %d must be 128, 256 or 512", t->block);
+  if (t->block != 256 && t->block != 512)
+    return fail(CFD_E_ARG, "cfd_set_tuning: block %d must be 256 or 512", t->block);
+  if (t->layout != kLayoutSoA && t->layout != kLayoutBlocked)
+    return fail(CFD_E_ARG, "cfd_set_tuning: layout %d must be 0 (SoA) or 1 (blocked)",
+                t->layout);
   if (t->slice_pad < 0 || t->slice_pad > (1 << 20))
     return fail(CFD_E_ARG, "cfd_set_tuning: slice_pad %d outside [0, 2^20]", t->slice_pad);
-  if (t->slice_pad != s->tuning.slice_pad) {
-    // Re-lay both population sets with the new slice stride.
-    const Geom ng = make_geom(s->nx, s->ny, s->nz, (uint32_t)t->slice_pad);
-    const size_t eb = elem_bytes(s);
-    const size_t nbytes = (size_t)Q * ng.S * eb;
+  const size_t eb = elem_bytes(s);
+  const Geom ng = make_geom(s->nx, s->ny, s->nz, t->layout, (uint32_t)t->slice_pad);
+  if (t->layout == kLayoutBlocked && ng.pop_elems * eb >= (1ull << 32))
+    return fail(CFD_E_ARG, "cfd_set_tuning: the blocked layout needs a population set below "
+                           "4 GiB (this grid: %.2f GB)", ng.pop_elems * eb / 1e9);
+  if (ng.lb != s->geom.lb || ng.S != s->geom.S) {
+    // Re-lay both population sets (contents preserved, walls included).
+    const size_t n = (size_t)Q * s->N;
+    float* dense = nullptr;
     void* np[2] = {nullptr, nullptr};
-    for (int k = 0; k < 2; ++k) {
-      hipError_t e = hipMalloc(&np[k], nbytes);
+    const size_t nbytes = ng.pop_elems * eb;
+    hipError_t e = hipMalloc(&dense, 2 * n * sizeof(float));
+    for (int k = 0; k < 2 && e == hipSuccess; ++k)
+      e = launch_store_to_f32(s->store, s->pop[k], dense + k * n, s->geom, s->stream);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+      e = hipMalloc(&np[k], nbytes);
       if (e == hipSuccess) e = hipMemsetAsync(np[k], 0, nbytes, s->stream);
-      if (e == hipSuccess)
-        e = hipMemcpy2DAsync(np[k], (size_t)ng.S * eb, s->pop[k], (size_t)s->geom.S * eb,
-                             s->N * eb, Q, hipMemcpyDeviceToDevice, s->stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-      if (e != hipSuccess) {
-        for (void* p : np)
-          if (p) (void)hipFree(p);
-        (void)hipGetLastError();
-        return fail(e == hipErrorOutOfMemory ? CFD_E_OOM : CFD_E_HIP,
-                    "cfd_set_tuning: re-layout failed: %s", hipGetErrorString(e));
-      }
+      if (e == hipSuccess) e = launch_f32_to_store(s->store, dense + k * n, np[k], ng, s->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (dense) (void)hipFree(dense);
+    if (e != hipSuccess) {
+      for (void* p : np)
+        if (p) (void)hipFree(p);
+      (void)hipGetLastError();
+      return fail(e == hipErrorOutOfMemory ? CFD_E_OOM : CFD_E_HIP,
+                  "cfd_set_tuning: re-layout failed: %s", hipGetErrorString(e));
     }
     for (int k = 0; k < 2; ++k) {
       (void)hipFree(s->pop[k]);
@@ -338,7 +353,7 @@ int cfd_set_tuning(cfd_solver* s, const cfd_tuning* t) {
   }
   s->tuning.block = t->block;
   s->tuning.remap = t->remap ? 1 : 0;
-  s->tuning.nontemporal = t->nontemporal ? 1 : 0;
+  s->tuning.layout = t->layout;
   s->tuning.slice_pad = t->slice_pad;
   return CFD_OK;
 }
@@ -348,7 +363,7 @@ int cfd_get_tuning(cfd_solver* s, cfd_tuning* t) {
   if (!s || !t) return fail(CFD_E_ARG, "cfd_get_tuning: null argument");
   t->block = s->tuning.block;
   t->remap = s->tuning.remap;
-  t->nontemporal = s->tuning.nontemporal;
+  t->layout = s->tuning.layout;
   t->slice_pad = s->tuning.slice_pad;
   return CFD_OK;
 }

@@ -12,9 +12,18 @@ constexpr int kXcds = 8;     // MI355X: workgroups go round-robin over 8 XCDs
 
 // Grid geometry plus the magic numbers of the two divisions that turn a
 // linear cell index into (x, y, z) without integer division instructions.
+constexpr int kLayoutSoA = 0;      // pop[i*S + cell]
+constexpr int kLayoutBlocked = 1;  // pop[cell >> LB][i][cell & (2^LB - 1)]
+#ifndef CFD_BLOCK_LOG2
+#define CFD_BLOCK_LOG2 9
+#endif
+constexpr int kBlockLog2 = CFD_BLOCK_LOG2;  // 2^9 = 512-cell blocks
+
 struct Geom {
   uint32_t nx, ny, nz, nxny, N;
-  uint32_t S;           // elements between population slices (S >= N)
+  uint32_t S;           // SoA: elements between population slices (S >= N)
+  int lb;               // layout exponent: 0 = SoA, kBlockLog2 = blocked
+  size_t pop_elems;     // elements of one population set
   uint32_t mnx, snx;    // n / nx   == (umulhi(n, mnx) + n) >> snx
   uint32_t mnxy, snxy;  // n / nxny == (umulhi(n, mnxy) + n) >> snxy
   uint32_t nblocks;     // logical workgroups covering N cells
@@ -36,14 +45,14 @@ struct StepParams {
   float tinf[19];   // free-stream equilibrium of the sponge  (ir.js:464-475)
 };
 
-Geom make_geom(int nx, int ny, int nz, uint32_t slice_pad);
+Geom make_geom(int nx, int ny, int nz, int layout, uint32_t slice_pad);
 
 // Launch shape of the step kernel (chosen by the autotuner or the caller).
 struct Tuning {
-  int block;        // threads per workgroup: 128, 256 or 512
+  int block;        // threads per workgroup: 256 or 512
   int remap;        // 1: contiguous slab of workgroups per XCD
-  int nontemporal;  // 1: streaming (non-temporal) population loads/stores
-  int slice_pad;    // extra elements between population slices
+  int layout;       // kLayoutSoA or kLayoutBlocked
+  int slice_pad;    // SoA only: extra elements between population slices
 };
 
 // Population storage: float32 (152 B/cell/step) or scaled float16 (76 B).

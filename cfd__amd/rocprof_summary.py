@@ -1,0 +1,131 @@
+"""Summarize rocprofv3 CSV output into the short markdown kept under profiles/.
+
+Inputs are the directories written by three runs of the same command:
+  rocprofv3 --kernel-trace --stats -d <trace> -o run --output-format csv -- <cmd>
+  rocprofv3 --pmc FETCH_SIZE      -d <fetch> -o run --output-format csv -- <cmd>
+  rocprofv3 --pmc WRITE_SIZE      -d <write> -o run --output-format csv -- <cmd>
+
+FETCH_SIZE / WRITE_SIZE (KiB per dispatch) are calibrated against the copy
+kernel k_copy that the same command runs on a known byte count
+(cfd_copy_bandwidth, read = write = copy_bytes per dispatch): the factor is
+known bytes / counter, applied to the step kernel's counters.
+
+usage: python -m cfd__amd.rocprof_summary --trace D --fetch D --write D
+           --kernel k_step --cells N --bytes-per-cell 152 --copy-bytes 1073741824
+           --command "..." --out profiles/x.md
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+from collections import defaultdict
+from pathlib import Path
+
+
+def _stats(trace_dir: Path):
+    rows = list(csv.DictReader(open(trace_dir / "run_kernel_stats.csv")))
+    return [(r["Name"], int(r["Calls"]), float(r["AverageNs"]), float(r["MinNs"]),
+             float(r["MaxNs"]), float(r["Percentage"])) for r in rows]
+
+
+def _counter(d: Path, name: str):
+    agg = defaultdict(list)
+    for r in csv.DictReader(open(d / "run_counter_collection.csv")):
+        if r["Counter_Name"] == name:
+            agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in agg.items()}
+
+
+def _pick(d: dict, key: str):
+    for k, v in d.items():
+        if key in k:
+            return k, v
+    raise KeyError(key)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--trace", type=Path, required=True)
+    ap.add_argument("--fetch", type=Path, required=True)
+    ap.add_argument("--write", type=Path, required=True)
+    ap.add_argument("--kernel", default="k_step")
+    ap.add_argument("--cells", type=int, required=True)
+    ap.add_argument("--bytes-per-cell", type=int, default=152)
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--command", required=True)
+    ap.add_argument("--title", default="")
+    ap.add_argument("--out", type=Path, required=True)
+    a = ap.parse_args()
+
+    st = _stats(a.trace)
+    fetch, write = _counter(a.fetch, "FETCH_SIZE"), _counter(a.write, "WRITE_SIZE")
+    _, cf = _pick(fetch, "k_copy")
+    _, cw = _pick(write, "k_copy")
+    kib = a.copy_bytes / 1024
+    ff, fw = kib / cf, kib / cw
+    # the step kernel without the macro store is the one timed per step
+    step_rows = [r for r in st if a.kernel in r[0]]
+    main_row = max(step_rows, key=lambda r: r[1])
+    kname = main_row[0]
+    # the counter entry of the same instantiation as the trace's main row
+    # (the macro-writing variant has `true` in its template arguments)
+    def same(dct):
+        best = None
+        for k, v in dct.items():
+            if a.kernel in k and ("true" in k) == ("true" in kname):
+                best = v
+        return best
+    fk, wk = same(fetch), same(write)
+    traffic = (fk * ff + wk * fw) * 1024
+    alg = a.bytes_per_cell * a.cells
+    avg_s = main_row[2] * 1e-9
+    lines = [
+        f"# {a.title or a.kernel}",
+        "",
+        "Commands (run from the repository root on one MI355X):",
+        "",
+        "```",
+        f"rocprofv3 --kernel-trace --stats -d <trace> -o run --output-format csv -- {a.command}",
+        f"rocprofv3 --pmc FETCH_SIZE -d <fetch> -o run --output-format csv -- {a.command}",
+        f"rocprofv3 --pmc WRITE_SIZE -d <write> -o run --output-format csv -- {a.command}",
+        f"python -m cfd__amd.rocprof_summary --trace <trace> --fetch <fetch> --write <write> "
+        f"--kernel {a.kernel} --cells {a.cells} --bytes-per-cell {a.bytes_per_cell} "
+        f"--copy-bytes {a.copy_bytes} --command '...' --out {a.out}",
+        "```",
+        "",
+        "## Kernel trace (--stats)",
+        "",
+        "| kernel | calls | avg us | min us | max us | % time |",
+        "|---|---|---|---|---|---|",
+    ]
+    for name, calls, avg, mn, mx, pct in sorted(st, key=lambda r: -r[5])[:12]:
+        short = name.split("(")[0][:90]
+        lines.append(f"| `{short}` | {calls} | {avg / 1e3:.2f} | {mn / 1e3:.2f} | "
+                     f"{mx / 1e3:.2f} | {pct:.2f} |")
+    lines += [
+        "",
+        "## HBM traffic (--pmc, per dispatch)",
+        "",
+        f"- copy-kernel calibration: k_copy moves {a.copy_bytes} B each way; FETCH_SIZE reads "
+        f"{cf:.1f} KiB -> factor {ff:.4f}; WRITE_SIZE reads {cw:.1f} KiB -> factor {fw:.4f}",
+        f"- {a.kernel}: FETCH_SIZE {fk:.1f} KiB, WRITE_SIZE {wk:.1f} KiB (raw)",
+        f"- {a.kernel}: calibrated read {fk * ff * 1024 / 1e6:.1f} MB + write "
+        f"{wk * fw * 1024 / 1e6:.1f} MB per launch",
+        "",
+        "## Roofline",
+        "",
+        f"- algorithmic_bytes_per_launch: {alg}  ({a.bytes_per_cell} B/cell x {a.cells} cells)",
+        f"- traffic_bytes_per_launch: {traffic:.0f}",
+        f"- traffic / algorithmic: {traffic / alg:.4f}",
+        f"- avg_kernel_us: {main_row[2] / 1e3:.2f}  (main instantiation, {main_row[1]} calls)",
+        f"- achieved_algorithmic_GBps: {alg / avg_s / 1e9:.1f}",
+        f"- fraction_of_8TBps_peak: {alg / avg_s / 8e12:.4f}",
+        "",
+    ]
+    a.out.parent.mkdir(parents=True, exist_ok=True)
+    a.out.write_text("\n".join(lines))
+    print(a.out.read_text())
+
+
+if __name__ == "__main__":
+    main()

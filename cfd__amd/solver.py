@@ -184,17 +184,18 @@ class SolverMI355X:
 
     @property
     def ajuste(self) -> dict:
-        """Launch tuning of the step kernel (block, remap, nontemporal, slice_pad)."""
+        """Launch tuning and population layout of the step kernel
+        (block, remap, layout 0 = SoA / 1 = 512-cell blocks, slice_pad)."""
         t = _lib.Tuning()
         check(lib.cfd_get_tuning(self._h, C.byref(t)))
-        return {"block": t.block, "remap": t.remap, "nontemporal": t.nontemporal,
+        return {"block": t.block, "remap": t.remap, "layout": t.layout,
                 "slice_pad": t.slice_pad}
 
     @ajuste.setter
     def ajuste(self, v: dict) -> None:
         cur = self.ajuste
         cur.update(v)
-        t = _lib.Tuning(int(cur["block"]), int(cur["remap"]), int(cur["nontemporal"]),
+        t = _lib.Tuning(int(cur["block"]), int(cur["remap"]), int(cur["layout"]),
                         int(cur["slice_pad"]))
         check(lib.cfd_set_tuning(self._h, C.byref(t)))
 

@@ -64,10 +64,12 @@ typedef struct cfd_params {
 /* Launch shape and layout of the step kernel.  No reference counterpart:
  * the WGSL path has one fixed 64x1x1 workgroup (wgsl.js:186). */
 typedef struct cfd_tuning {
-  int32_t block;        /* threads per workgroup: 128, 256 or 512              */
+  int32_t block;        /* threads per workgroup: 256 or 512                   */
   int32_t remap;        /* 1: each XCD streams one contiguous slab of cells    */
-  int32_t nontemporal;  /* 1: non-temporal population loads and stores         */
-  int32_t slice_pad;    /* extra elements between the 19 population slices     */
+  int32_t layout;       /* 0: SoA pop[i][cell]; 1: 512-cell blocks
+                           pop[cell/512][i][cell%512] (default; needs a
+                           population set below 4 GiB, else SoA is used)        */
+  int32_t slice_pad;    /* SoA only: extra elements between the 19 slices      */
 } cfd_tuning;
 
 typedef struct cfd_vox_stats {
@@ -122,9 +124,9 @@ int cfd_get_front(cfd_solver *s, int *front);
  * the n steps (pulled, pre-collision moments; zero in solids).  Non-blocking. */
 int cfd_step(cfd_solver *s, int n);
 
-/* Launch tuning of the step kernel (autotuner hook; no reference
- * counterpart).  Changing slice_pad re-lays both population sets in place
- * (contents preserved).  Results do not depend on the tuning. */
+/* Launch tuning and population layout of the step kernel (autotuner hook;
+ * no reference counterpart).  Changing layout or slice_pad re-lays both
+ * population sets (contents preserved).  Results do not depend on it. */
 int cfd_set_tuning(cfd_solver *s, const cfd_tuning *t);
 int cfd_get_tuning(cfd_solver *s, cfd_tuning *t);
 
