@@ -21,9 +21,11 @@ def _S():
     return SolverMI355X
 
 
-@pytest.mark.parametrize("precisao,tol", [("fp32", 1e-6), ("fp16", 1e-6)])
+@pytest.mark.parametrize("precisao,tol", [("fp32", 1e-6), ("fp16", 5e-5)])
 def test_uniform_flow_preserved(precisao, tol):
-    """nucleo.html:107-130 (tolerance 1e-6; README quotes 1.9e-8)."""
+    """nucleo.html:107-130 (tolerance 1e-6; README quotes 1.9e-8).  With
+    float16 storage of g_i = f_i - w_i the stream is held to the storage
+    quantum instead: |g| ~ 8e-3 carries a half-ulp of 4e-6, about 1e-5 in u."""
     s = _S().criar({"nx": 64, "ny": 32, "nz": 32}, precisao=precisao)
     try:
         s.definirTipos(np.zeros(s.N, np.uint32))

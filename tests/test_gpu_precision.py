@@ -1,0 +1,133 @@
+"""Reduced-precision storage against the float64 CPU oracle.
+
+Populations stored as float16 deviations g_i = f_i - w_i with collision in
+float32, and the float32 path itself, are compared with the float64
+restatement of the same algorithm (oracle/lbm_oracle.c):
+
+  * shear-wave decay on 128^3 (amplitude 1e-3, omega 0.6 ... 1.8): the
+    measured viscosity against the float64 run (an 8x128x8 grid gives the
+    float64 decay exactly: the wave is independent of x and z);
+  * the 160x80x64 sphere tunnel: Cd and mass drift after 1000 steps against
+    the float64 oracle run of the same case.
+
+The numbers are printed; the asserted bounds are loose sanity limits.
+"""
+import json
+import math
+
+import numpy as np
+import pytest
+
+from cfd__amd import lattice, meshes, tunel
+
+pytestmark = pytest.mark.gpu
+
+
+def _nu_from(mx, ny, T, A0):
+    k = 2 * math.pi / ny
+    sn = np.sin(k * np.arange(ny))
+    amp = (mx.mean(axis=(0, 2)) * sn).sum() / (sn * sn).sum()
+    return -math.log(abs(amp) / A0) / (k * k * T)
+
+
+def _gpu_shear(omega, precisao, n=128, T=400, A0=1e-3):
+    from cfd__amd.solver import SolverMI355X
+    k = 2 * math.pi / n
+    s = SolverMI355X.criar({"nx": n, "ny": n, "nz": n}, precisao=precisao)
+    try:
+        s.definirTipos(np.zeros(s.N, np.uint32))
+        # no sponge: the wave is periodic in x (the default sponge would damp
+        # the last 18 % of the box toward rest)
+        s.configurar(omegaPlus=omega, lesCs=0.0, inletU=[0, 0, 0],
+                     esponja={"inicio": 1e9, "comprimento": 1})
+        s.frente = "A"
+        s.definirCampo(lambda X, Y, Z: {"u": [A0 * np.sin(k * Y), 0.0, 0.0]}, vetorizado=True)
+        s.passo(T)
+        s.atualizarMacros()
+        m = s.lerMacros().reshape(n, n, n, 4)[..., 0].astype(np.float64)
+        mass, _ = s.massa()
+    finally:
+        s.destruir()
+    return _nu_from(m, n, T, A0), mass
+
+
+def _oracle_shear(orc, omega, n=128, T=400, A0=1e-3):
+    k = 2 * math.pi / n
+    grid = (8, n, 8)
+    Z, Y, X = np.meshgrid(np.arange(8), np.arange(n), np.arange(8), indexing="ij")
+    ux = A0 * np.sin(k * Y)
+    pops = lattice.equilibrium(np.zeros_like(ux), [ux, 0 * ux, 0 * ux]).reshape(19, -1)
+    par = orc.params_array(omegaPlus=omega, lesCs=0.0, inletU=(0, 0, 0),
+                           esponja={"inicio": 1e9, "comprimento": 1}, nx=8)
+    _, m = orc.run(grid, pops, np.zeros(8 * n * 8, np.uint32), par, T, dtype=np.float64)
+    return _nu_from(m.reshape(8, n, 8, 4)[..., 0], n, T, A0)
+
+
+def test_shear_wave_viscosity_fp16_vs_fp64(orc):
+    rows = []
+    for omega in (0.6, 0.8, 1.0, 1.2, 1.5, 1.8):
+        nu64 = _oracle_shear(orc, omega)
+        nu32, m32 = _gpu_shear(omega, "fp32")
+        nu16, m16 = _gpu_shear(omega, "fp16")
+        rows.append({"omega": omega, "nu_theory": lattice.nuFromOmega(omega), "nu_fp64": nu64,
+                     "err_fp32_vs_fp64": (nu32 - nu64) / nu64,
+                     "err_fp16_vs_fp64": (nu16 - nu64) / nu64,
+                     "mass_fp32": m32, "mass_fp16": m16})
+    print("SHEAR_WAVE_128", json.dumps(rows))
+    for r in rows:
+        assert abs(r["err_fp32_vs_fp64"]) < 1e-3
+        assert math.isfinite(r["err_fp16_vs_fp64"])
+    # float16 storage rounds g_i to ~5e-4 relative every step; at amplitude
+    # 1e-3 that dominates the per-step viscous decay once omega > 1, so only
+    # the diffusive end is held to a bound
+    for r in rows:
+        if r["omega"] <= 1.0:
+            assert abs(r["err_fp16_vs_fp64"]) < 0.1
+
+
+def test_sphere_tunnel_cd_and_mass_drift_vs_fp64(orc):
+    """160x80x64, icosphere D=16 at (48,40,32), u=0.05, nu for Re=100,
+    mirror floor; 1000 steps."""
+    from cfd__amd.forces import coeficientes
+    from cfd__amd.solver import SolverMI355X
+    from cfd__amd.units import Units
+    from cfd__amd.voxel import voxelizar
+    nx, ny, nz = 160, 80, 64
+    g = {"nx": nx, "ny": ny, "nz": nz}
+    pos, idx = meshes.icosphere(5, 16.0, (48.0, 40.0, 32.0), 1)
+    corpo = voxelizar(pos, idx, g)["tipo"]
+    tipo, esp, _ = tunel.montarTipos(g, corpo, esteira=False)
+    t3 = tipo.reshape(nz, ny, nx)
+    t3[0, :, 1:nx - 1] = 6
+    nu = 0.05 * 16 / 100
+    omega = 1 / (3 * nu + 0.5)
+    cfg = dict(omegaPlus=omega, lesCs=0.1, inletU=[0.05, 0, 0], esponja=esp)
+    steps = 1000
+    area = tunel.areaFrontalCorpo(corpo, g)
+    units = Units(lengthM=0.016, speedMs=0.1, cellsPerLength=16, uLb=0.05, lesCs=0.1)
+    res = {}
+    for prec in ("fp32", "fp16"):
+        s = SolverMI355X.criar(g, precisao=prec)
+        try:
+            s.definirTipos(tipo)
+            s.configurar(**cfg)
+            s.inicializar()
+            m0, _ = s.massa()
+            s.passo(steps)
+            f = s.medirForcas()
+            m1, bad = s.massa()
+        finally:
+            s.destruir()
+        res[prec] = {"cd": coeficientes(f, units, area, g)["cd"], "mass0": m0, "mass1": m1,
+                     "bad": bad}
+    par = orc.params_array(nx=nx, **cfg)
+    pops = orc.init((nx, ny, nz), tipo, par, dtype=np.float64)
+    out, _ = orc.run((nx, ny, nz), pops, tipo, par, steps, dtype=np.float64)
+    f64 = orc.forces((nx, ny, nz), out, tipo, dtype=np.float64)
+    p = out.reshape(19, -1)[:, ~np.isin(tipo, (1, 4, 7))]
+    res["fp64"] = {"cd": coeficientes(list(f64), units, area, g)["cd"],
+                   "mass1": float(p.sum())}
+    print("SPHERE_160x80x64", json.dumps(res))
+    assert res["fp32"]["bad"] == 0 and res["fp16"]["bad"] == 0
+    assert abs(res["fp32"]["cd"] - res["fp64"]["cd"]) < 1e-3 * abs(res["fp64"]["cd"])
+    assert abs(res["fp16"]["cd"] - res["fp64"]["cd"]) < 0.05 * abs(res["fp64"]["cd"])
