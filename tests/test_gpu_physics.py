@@ -168,3 +168,10 @@ def test_validation_cases_against_correlations(caso):
     assert r["classificacao"]["ok"], r
     if "st" in r:
         assert r["stClassificacao"]["ok"], r
+    # the reference's own published results of the same cases (README.md:102-105),
+    # to the digits it prints
+    published = {"esfera-re50": (1.480, None), "esfera-re100": (1.070, None),
+                 "cilindro-re100": (1.285, 0.1663)}[caso["id"]]
+    assert abs(r["cd"] - published[0]) <= 0.0005 + 1e-9, r["cd"]
+    if published[1] is not None:
+        assert abs(r["st"] - published[1]) <= 0.00005 + 1e-9, r["st"]

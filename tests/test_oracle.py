@@ -320,3 +320,23 @@ def test_force_symmetry_and_drag_sign(orc):
     f = orc.forces((nx, ny, nz), out, t, dtype=np.float64)
     assert f[0] > 0
     assert abs(f[1]) < 1e-10 * f[0] and abs(f[2]) < 1e-10 * f[0]
+
+
+def test_init_and_macros_closed_form(orc):
+    """init = eq(0, u) with u = inletU, 0 in SOLIDO, beltU in SOLIDO_MOVEL
+    (wgsl.js:269-302); macros of an equilibrium returns its (u, delta)."""
+    grid = (4, 3, 2)
+    t = np.array([0, 1, 2, 3, 4, 5, 6, 7] * 3, dtype=np.uint32)
+    inlet, belt = (0.05, -0.01, 0.02), (0.03, 0.0, 0.0)
+    par = orc.params_array(omegaPlus=1.0, inletU=inlet, beltU=belt, nx=4)
+    pops = orc.init(grid, t, par, dtype=np.float64).reshape(Q, -1)
+    for c, tt in enumerate(t):
+        u = (0, 0, 0) if tt == 1 else (belt if tt == 4 else inlet)
+        assert np.abs(pops[:, c] - lattice.equilibrium(0.0, u)).max() < 1e-17
+    d = np.linspace(-1e-3, 1e-3, 24)
+    u = [np.full(24, 0.04), np.linspace(-0.01, 0.01, 24), np.full(24, -0.02)]
+    m = orc.macros(grid, lattice.equilibrium(d, u).reshape(Q, -1), dtype=np.float64)
+    m = m.reshape(-1, 4)
+    assert np.abs(m[:, 3] - d).max() < 1e-16
+    for a in range(3):
+        assert np.abs(m[:, a] - u[a]).max() < 1e-15
