@@ -38,6 +38,16 @@ class Tuning(C.Structure):
                 ("slice_pad", C.c_int32)]
 
 
+class TunnelSpec(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("tipo", C.c_void_p), ("params", Params)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("block", C.c_int32), ("tile_cells", C.c_int32), ("wg_per_cu", C.c_int32),
+                ("max_active_tiles", C.c_int32)]
+
+
 class VoxStats(C.Structure):
     _fields_ = [
         ("n_cells", C.c_int64), ("n_casca", C.c_int64), ("n_dentro", C.c_int64),
@@ -77,6 +87,14 @@ def _load() -> C.CDLL:
                                    C.c_double, C.c_int, P, P, C.POINTER(VoxStats)]),
         "cfd_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]),
+        "cfd_batch_create": (C.c_int, [P, C.c_int, C.POINTER(P)]),
+        "cfd_batch_reset": (C.c_int, [P]),
+        "cfd_batch_run": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(BatchTuning),
+                                    C.POINTER(C.c_double)]),
+        "cfd_batch_results": (C.c_int, [P, P, P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "cfd_batch_get_populations": (C.c_int, [P, C.c_int, P, C.c_size_t]),
+        "cfd_batch_max_grid": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "cfd_batch_destroy": (None, [P]),
         "cfd_last_error": (C.c_char_p, []),
         "cfd_version": (C.c_char_p, []),
     }
@@ -96,6 +114,8 @@ EXPORTS = (
     "cfd_step", "cfd_step_timed", "cfd_set_tuning", "cfd_get_tuning", "cfd_forces", "cfd_update_macros", "cfd_read_macros",
     "cfd_mass", "cfd_synchronize", "cfd_bytes", "cfd_destroy", "cfd_voxelize",
     "cfd_copy_bandwidth", "cfd_last_error", "cfd_version",
+    "cfd_batch_create", "cfd_batch_reset", "cfd_batch_run", "cfd_batch_results",
+    "cfd_batch_get_populations", "cfd_batch_max_grid", "cfd_batch_destroy",
 )
 
 

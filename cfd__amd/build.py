@@ -34,10 +34,15 @@ HIP_FLAGS = [
 ]
 # Per-file extras: the voxelizer must not contract a*b+c into fma, since it
 # reproduces IEEE float64 JavaScript arithmetic bit for bit.
+# The lattice kernels round every product before the following add, as the
+# reference's shader text is written: every instantiation of the step (with or
+# without the macro store, the force accumulation or the batch loop around it)
+# then rounds identically, and identically to a plain IEEE float32 CPU build.
 FILE_FLAGS = {
     "voxel.hip": ["-ffp-contract=off"],
+    "lbm_kernels.hip": ["-ffp-contract=off"],
 }
-SOURCES = ["lbm_kernels.hip", "cfd_core.hip", "voxel.hip"]
+SOURCES = ["lbm_kernels.hip", "cfd_core.hip", "batch_api.hip", "voxel.hip"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

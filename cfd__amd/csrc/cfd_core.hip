@@ -106,6 +106,57 @@ size_t elem_bytes(const cfd_solver* s) { return s->store == Store::F16 ? 2 : 4; 
 
 }  // namespace
 
+namespace cfd {
+
+// The Params uniform (solver.js:277-293) rounded to float32 as
+// Float32Array.set() does, plus the uniform sub-expressions of the emitted
+// shader folded once, in float32 and in the shader's operation order
+// (volatile keeps each rounding where the shader has it).
+StepParams make_step_params(const cfd_params& p) {
+  StepParams q{};
+  q.omega_plus = (float)p.omega_plus;
+  q.magic = (float)p.magic;
+  q.les_cs = (float)p.les_cs;
+  for (int a = 0; a < 3; ++a) {
+    q.belt[a] = (float)p.belt[a];
+    q.inlet[a] = (float)p.inlet[a];
+  }
+  q.sponge_start = (float)p.sponge_start;
+  const float len = (float)p.sponge_len;
+  q.sponge_len_m = len > 1.0f ? len : 1.0f;  // max(uSpongeLen, 1.0)  (ir.js:461)
+  q.bl = (float)p.bl_thickness;
+  q.bl_m = q.bl > 1.0f ? q.bl : 1.0f;         // max(P.sponge.z, 1.0) (wgsl.js:217)
+  volatile float om = q.omega_plus;
+  q.tau0 = 1.0f / om;                         // ir.js:290
+  volatile float t0 = q.tau0;
+  q.tau0sq = t0 * t0;                         // ir.js:292
+  volatile float cs = q.les_cs;
+  volatile float c18 = 18.0f * cs;
+  q.c18cs2 = c18 * cs;                        // ir.js:293
+  // free-stream equilibrium of the sponge (ir.js:464-475)
+  const float* ui = q.inlet;
+  volatile float uu_a = ui[0] * ui[0];
+  volatile float uu_b = uu_a + ui[1] * ui[1];
+  volatile float uuinf = uu_b + ui[2] * ui[2];
+  {
+    volatile float m15 = -1.5f * uuinf;
+    q.tinf[0] = wf(0) * m15;
+  }
+  for (int i = 1; i < Q; ++i) {
+    volatile float si = dotc_host(i, ui);
+    volatile float a = 3.0f * si;
+    volatile float b = 4.5f * si;
+    volatile float b2 = b * si;
+    volatile float ab = a + b2;
+    volatile float c = 1.5f * uuinf;
+    volatile float r = ab - c;
+    q.tinf[i] = wf(i) * r;
+  }
+  return q;
+}
+
+}  // namespace cfd
+
 extern "C" {
 
 const char* cfd_last_error(void) { return g_err.c_str(); }
@@ -191,46 +242,7 @@ int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
 int cfd_configure(cfd_solver* s, const cfd_params* p) {
   g_err.clear();
   if (!s || !p) return fail(CFD_E_ARG, "cfd_configure: null argument");
-  StepParams q{};
-  q.omega_plus = (float)p->omega_plus;
-  q.magic = (float)p->magic;
-  q.les_cs = (float)p->les_cs;
-  for (int a = 0; a < 3; ++a) {
-    q.belt[a] = (float)p->belt[a];
-    q.inlet[a] = (float)p->inlet[a];
-  }
-  q.sponge_start = (float)p->sponge_start;
-  const float len = (float)p->sponge_len;
-  q.sponge_len_m = len > 1.0f ? len : 1.0f;  // max(uSpongeLen, 1.0)
-  q.bl = (float)p->bl_thickness;
-  q.bl_m = q.bl > 1.0f ? q.bl : 1.0f;         // max(P.sponge.z, 1.0)
-  // Uniform sub-expressions, evaluated in float32 in the shader's order.
-  volatile float om = q.omega_plus;
-  q.tau0 = 1.0f / om;
-  volatile float t0 = q.tau0;
-  q.tau0sq = t0 * t0;
-  volatile float cs = q.les_cs;
-  volatile float c18 = 18.0f * cs;
-  q.c18cs2 = c18 * cs;
-  // Free-stream equilibrium of the sponge (ir.js:464-475), float32.
-  const float* ui = q.inlet;
-  volatile float uu_a = ui[0] * ui[0];
-  volatile float uu_b = uu_a + ui[1] * ui[1];
-  volatile float uuinf = uu_b + ui[2] * ui[2];
-  {
-    volatile float m15 = -1.5f * uuinf;
-    q.tinf[0] = wf(0) * m15;
-  }
-  for (int i = 1; i < Q; ++i) {
-    volatile float si = dotc_host(i, ui);
-    volatile float a = 3.0f * si;
-    volatile float b = 4.5f * si;
-    volatile float b2 = b * si;
-    volatile float ab = a + b2;
-    volatile float c = 1.5f * uuinf;
-    volatile float r = ab - c;
-    q.tinf[i] = wf(i) * r;
-  }
+  const StepParams q = make_step_params(*p);
   s->params = q;
   s->configured = true;
   return CFD_OK;

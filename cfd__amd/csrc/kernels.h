@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "../../include/cfd_mi355x.h"
+
 namespace cfd {
 
 constexpr int kBlock = 256;  // threads per workgroup of the cell kernels
@@ -47,6 +49,9 @@ struct StepParams {
 
 Geom make_geom(int nx, int ny, int nz, int layout, uint32_t slice_pad);
 
+struct StepParams;
+StepParams make_step_params(const cfd_params& p);
+
 // Launch shape of the step kernel (chosen by the autotuner or the caller).
 struct Tuning {
   int block;        // threads per workgroup: 256 or 512
@@ -80,5 +85,30 @@ hipError_t launch_f32_to_store(Store sto, const float* in, void* out, const Geom
 hipError_t launch_store_to_f32(Store sto, const void* in, float* out, const Geom& g,
                                hipStream_t st);
 hipError_t launch_copy(const float4* in, float4* out, size_t n4, hipStream_t st);
+
+// ---- batched tunnels (one cooperative launch) ----
+struct BatchTunnel {       // one tunnel: geometry, uniforms, buffers
+  Geom g;
+  StepParams p;
+  void* pop[2];            // float32, blocked layout; step s reads pop[s & 1]
+  const uint8_t* tipo8;
+};
+struct BatchPhase {        // rounds [r0, r1) with a fixed admitted set
+  uint32_t r0, r1;
+  uint32_t first;          // index of its first BatchEntry
+  uint32_t n_active;
+  uint32_t total_tiles;
+};
+struct BatchEntry {        // one admitted tunnel inside a phase
+  uint32_t tunnel;
+  uint32_t start_round;    // round of its first step
+  uint32_t tile_prefix;    // first tile index of this tunnel in the phase
+  uint32_t pad;
+};
+hipError_t batch_max_grid(int block, int* max_blocks_per_cu, int* cus);
+hipError_t launch_batch_run(const BatchTunnel* tun, const BatchPhase* phases, int nphases,
+                            const BatchEntry* entries, int tile_cells, int sample_every,
+                            int max_samples, double* samples, unsigned long long* clocks,
+                            int block, int grid, hipStream_t st);
 
 }  // namespace cfd

@@ -19,6 +19,7 @@
 //  * the cell type is one byte with two precomputed flags (near-wall, body
 //    link), so the 18 neighbour-type reads of the bounce-back test happen only
 //    for cells that touch a wall.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -185,14 +186,17 @@ __global__ __launch_bounds__(kBlock) void k_prep_types(const uint32_t* __restric
 // pull streaming with halfway bounce-back at the fluid node, moments, then
 // the branch chain ESPELHO_Y -> ESPELHO_Z -> ENTRADA -> SAIDA -> fluid
 // (equilibrium, closed-form Smagorinsky, TRT per opposite pair, sponge).
-template <class S, bool MACROS, int BLOCK, int LB>
-__global__ __launch_bounds__(BLOCK) void k_step(const typename S::T* __restrict__ src,
-                                                typename S::T* __restrict__ dst,
-                                                const uint8_t* __restrict__ tipo,
-                                                float4* __restrict__ macros, Geom g,
-                                                StepParams p, uint32_t per_xcd, int remap) {
-  const uint32_t cell = logical_block(per_xcd, remap) * BLOCK + threadIdx.x;
-  if (cell >= g.N) return;
+//
+// FORCE = true also accumulates, into f, the momentum exchange of the state
+// being pulled (the previous step's post-collision populations): every pull
+// of a FLUIDO cell from a SOLIDO neighbour x + c_j re-reads g_j(x), which is
+// exactly the link term c_j * 2 (g_j + w_j) of wgsl.js:358-379.
+template <class S, bool MACROS, int LB, bool FORCE>
+__device__ __forceinline__ void step_cell(uint32_t cell, const typename S::T* __restrict__ src,
+                                          typename S::T* __restrict__ dst,
+                                          const uint8_t* __restrict__ tipo,
+                                          float4* __restrict__ macros, const Geom& g,
+                                          const StepParams& p, double* f) {
   const Coord c = split(cell, g);
   const Wrap w = wrap_offsets(c, g);
 
@@ -229,7 +233,15 @@ __global__ __launch_bounds__(BLOCK) void k_step(const typename S::T* __restrict_
         const bool moving = (t == SOLIDO_MOVEL);
         const float cu = dotc(i, moving ? p.belt[0] : 0.f, moving ? p.belt[1] : 0.f,
                               moving ? p.belt[2] : 0.f);
-        gi[i] = S::ld(pop_at<LB>(src, opp(i), cell, g)) + kwall(i) * cu;
+        const float back = S::ld(pop_at<LB>(src, opp(i), cell, g));
+        gi[i] = back + kwall(i) * cu;
+        if (FORCE && t == SOLIDO && ct == FLUIDO) {
+          const int j = opp(i);
+          const double q = 2.0 * ((double)back + wd(j));
+          if (cx(j)) f[0] += cx(j) > 0 ? q : -q;
+          if (cy(j)) f[1] += cy(j) > 0 ? q : -q;
+          if (cz(j)) f[2] += cz(j) > 0 ? q : -q;
+        }
       }
     }
   }
@@ -352,6 +364,112 @@ __global__ __launch_bounds__(BLOCK) void k_step(const typename S::T* __restrict_
 #pragma unroll
   for (int i = 0; i < Q; ++i) S::st(pop_at<LB>(dst, i, cell, g), d[i]);
   if (MACROS) macros[cell] = make_float4(ux, uy, uz, delta);
+}
+
+template <class S, bool MACROS, int BLOCK, int LB>
+__global__ __launch_bounds__(BLOCK) void k_step(const typename S::T* __restrict__ src,
+                                                typename S::T* __restrict__ dst,
+                                                const uint8_t* __restrict__ tipo,
+                                                float4* __restrict__ macros, Geom g,
+                                                StepParams p, uint32_t per_xcd, int remap) {
+  const uint32_t cell = logical_block(per_xcd, remap) * BLOCK + threadIdx.x;
+  if (cell >= g.N) return;
+  step_cell<S, MACROS, LB, false>(cell, src, dst, tipo, macros, g, p, nullptr);
+}
+
+// Workgroup sum of 3 doubles for any BLOCK (wave64 butterfly, then the
+// waves in fixed order through LDS); the result is valid in thread 0.
+template <int BLOCK>
+__device__ __forceinline__ void block_sum_into(double (&v)[3], double (&out)[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  }
+  __shared__ double sh[BLOCK / 64][3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 3; ++k) sh[wave][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) {
+      double s = sh[0][k];
+      for (int q = 1; q < BLOCK / 64; ++q) s += sh[q][k];
+      out[k] = s;
+    }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ batch
+
+// Many independent tunnels in ONE cooperative launch.  The host builds a
+// schedule of rounds: in round r every admitted tunnel advances one step,
+// its cells split into tiles of `tile_cells` consecutive cells; the tiles of
+// all admitted tunnels form one flat list that the workgroups stride over,
+// then the whole grid synchronizes (all workgroups are co-resident: the
+// runtime refuses a cooperative launch that does not fit).  Rounds with the
+// same admitted set form a phase.  A tunnel's parity (A or B as source) is
+// its own step count, so tunnels admitted at different rounds coexist.
+// Sampled steps also accumulate the body force (float64 per workgroup, one
+// atomic add per workgroup and component into the tunnel's sample slot).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch_run(const BatchTunnel* __restrict__ tun,
+                                                     const BatchPhase* __restrict__ phases,
+                                                     int nphases,
+                                                     const BatchEntry* __restrict__ entries,
+                                                     int tile_cells, int sample_every,
+                                                     int max_samples,
+                                                     double* __restrict__ samples,
+                                                     unsigned long long* __restrict__ clocks) {
+  namespace cg = cooperative_groups;
+  cg::grid_group grid = cg::this_grid();
+  if (blockIdx.x == 0 && threadIdx.x == 0) clocks[0] = wall_clock64();
+  for (int ph = 0; ph < nphases; ++ph) {
+    const BatchPhase P = phases[ph];
+    for (uint32_t r = P.r0; r < P.r1; ++r) {
+      for (uint32_t t = blockIdx.x; t < P.total_tiles; t += gridDim.x) {
+        // the admitted tunnel owning tile t (tile_prefix ascending)
+        uint32_t lo = 0, hi = P.n_active;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (entries[P.first + mid].tile_prefix <= t) lo = mid; else hi = mid;
+        }
+        const BatchEntry E = entries[P.first + lo];
+        const BatchTunnel& D = tun[E.tunnel];
+        const uint32_t step = r - E.start_round;
+        const float* src = static_cast<const float*>(D.pop[step & 1]);
+        float* dst = static_cast<float*>(D.pop[(step & 1) ^ 1]);
+        const uint32_t c0 = (t - E.tile_prefix) * (uint32_t)tile_cells;
+        const uint32_t c1 = min(c0 + (uint32_t)tile_cells, D.g.N);
+        const bool sample = sample_every > 0 && ((step + 1) % (uint32_t)sample_every) == 0;
+        double f[3] = {0.0, 0.0, 0.0};
+        if (sample) {
+          for (uint32_t cell = c0 + threadIdx.x; cell < c1; cell += BLOCK)
+            step_cell<StoreF32, false, kBlockLog2, true>(cell, src, dst, D.tipo8, nullptr, D.g,
+                                                         D.p, f);
+        } else {
+          for (uint32_t cell = c0 + threadIdx.x; cell < c1; cell += BLOCK)
+            step_cell<StoreF32, false, kBlockLog2, false>(cell, src, dst, D.tipo8, nullptr,
+                                                          D.g, D.p, f);
+        }
+        if (sample) {  // uniform over the workgroup
+          double part[3];
+          block_sum_into<BLOCK>(f, part);
+          if (threadIdx.x == 0) {
+            const int k = (int)((step + 1) / (uint32_t)sample_every) - 1;
+            if (k < max_samples) {
+              double* slot = samples + ((size_t)E.tunnel * max_samples + k) * 3;
+              atomicAdd(slot + 0, part[0]);
+              atomicAdd(slot + 1, part[1]);
+              atomicAdd(slot + 2, part[2]);
+            }
+          }
+        }
+      }
+      grid.sync();
+      if (blockIdx.x == 0 && threadIdx.x == 0) clocks[1 + r] = wall_clock64();
+    }
+  }
 }
 
 // ------------------------------------------------------------------ init
@@ -707,6 +825,32 @@ hipError_t launch_store_to_f32(Store sto, const void* in, float* out, const Geom
                        static_cast<const typename S::T*>(in), out, g);
   });
   return hipGetLastError();
+}
+
+hipError_t batch_max_grid(int block, int* max_blocks_per_cu, int* cus) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  if (block == 512)
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(max_blocks_per_cu, k_batch_run<512>, 512,
+                                                        0);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(max_blocks_per_cu, k_batch_run<256>, 256, 0);
+}
+
+hipError_t launch_batch_run(const BatchTunnel* tun, const BatchPhase* phases, int nphases,
+                            const BatchEntry* entries, int tile_cells, int sample_every,
+                            int max_samples, double* samples, unsigned long long* clocks,
+                            int block, int grid, hipStream_t st) {
+  void* args[] = {(void*)&tun,          (void*)&phases,      (void*)&nphases,
+                  (void*)&entries,      (void*)&tile_cells,  (void*)&sample_every,
+                  (void*)&max_samples,  (void*)&samples,     (void*)&clocks};
+  if (block == 512)
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_batch_run<512>),
+                                      dim3(grid), dim3(512), args, 0, st);
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_batch_run<256>), dim3(grid),
+                                    dim3(256), args, 0, st);
 }
 
 hipError_t launch_copy(const float4* in, float4* out, size_t n4, hipStream_t st) {

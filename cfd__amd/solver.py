@@ -59,20 +59,8 @@ class SolverMI355X:
                    espessuraCL: float = 0.0) -> None:
         """solver.js:277-293; esponja defaults to start round(0.82 nx), length
         max(4, round(0.18 nx))."""
-        esp = esponja if esponja is not None else {
-            "inicio": _js_round(self.nx * 0.82),
-            "comprimento": max(4, _js_round(self.nx * 0.18)),
-        }
-        p = Params()
-        p.omega_plus = omegaPlus
-        p.magic = magic
-        p.les_cs = lesCs
-        for a in range(3):
-            p.belt[a] = beltU[a]
-            p.inlet[a] = inletU[a]
-        p.sponge_start = esp["inicio"]
-        p.sponge_len = esp["comprimento"]
-        p.bl_thickness = espessuraCL
+        p, esp = params_struct(self.nx, omegaPlus, magic, lesCs, inletU, beltU, esponja,
+                               espessuraCL)
         check(lib.cfd_configure(self._h, C.byref(p)))
         self.cfg = dict(omegaPlus=omegaPlus, magic=magic, lesCs=lesCs, inletU=list(inletU),
                         beltU=list(beltU), esponja=esp, espessuraCL=espessuraCL)
@@ -247,6 +235,26 @@ class SolverMI355X:
             self.destruir()
         except Exception:
             pass
+
+
+def params_struct(nx, omegaPlus, magic=MAGIC["WALL"], lesCs=0.1, inletU=(0.0, 0.0, 0.0),
+                  beltU=(0.0, 0.0, 0.0), esponja=None, espessuraCL=0.0):
+    """cfd_params of configurar(); the sponge defaults as solver.js:281-284."""
+    esp = esponja if esponja is not None else {
+        "inicio": _js_round(nx * 0.82),
+        "comprimento": max(4, _js_round(nx * 0.18)),
+    }
+    p = Params()
+    p.omega_plus = omegaPlus
+    p.magic = magic
+    p.les_cs = lesCs
+    for a in range(3):
+        p.belt[a] = beltU[a]
+        p.inlet[a] = inletU[a]
+    p.sponge_start = esp["inicio"]
+    p.sponge_len = esp["comprimento"]
+    p.bl_thickness = espessuraCL
+    return p, esp
 
 
 def _js_round(x: float) -> int:

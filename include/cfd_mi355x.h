@@ -162,6 +162,50 @@ int cfd_bytes(cfd_solver *s, int64_t *bytes_out);
 /* SolverWebGPU.destruir (solver.js:523-527). */
 void cfd_destroy(cfd_solver *s);
 
+/* ---- batched tunnels: many independent float32 tunnels, one launch ----
+ * No reference counterpart (the reference runs one tunnel per page).  Each
+ * tunnel is a grid, a cell-type field and a parameter set; all tunnels share
+ * the kernel of cfd_step.  cfd_batch_run advances every tunnel by `steps`
+ * steps inside ONE cooperative launch: tunnels are admitted smallest first
+ * while the admitted tiles stay within max_active_tiles, each round steps
+ * every admitted tunnel once, then the grid synchronizes; a finished tunnel
+ * leaves and the next is admitted.  Force samples (momentum exchange of the
+ * state after every `sample_every` steps, float64) and each tunnel's
+ * completion time are kept for cfd_batch_results. */
+typedef struct cfd_batch cfd_batch;
+
+typedef struct cfd_tunnel_spec {
+  int32_t nx, ny, nz;
+  const uint32_t *tipo;   /* nx*ny*nz CELULA codes, copied at creation       */
+  cfd_params params;
+} cfd_tunnel_spec;
+
+typedef struct cfd_batch_tuning {
+  int32_t block;            /* threads per workgroup: 256 or 512             */
+  int32_t tile_cells;       /* consecutive cells per tile (64 .. 2^20)       */
+  int32_t wg_per_cu;        /* resident workgroups per CU; 0 = maximum       */
+  int32_t max_active_tiles; /* admission cap; 0 = 8 tiles per workgroup      */
+} cfd_batch_tuning;
+
+/* Allocates every tunnel (blocked float32 layout), uploads its types and
+ * initializes it to the free-stream equilibrium. */
+int cfd_batch_create(const cfd_tunnel_spec *specs, int n, cfd_batch **out);
+/* Back to the free-stream equilibrium (e.g. after autotuning runs). */
+int cfd_batch_reset(cfd_batch *b);
+/* One cooperative launch running `steps` steps of every tunnel; tuning may
+ * be NULL (last used / default).  ms_out: device time of the launch. */
+int cfd_batch_run(cfd_batch *b, int steps, int sample_every, const cfd_batch_tuning *t,
+                  double *ms_out);
+/* forces: n * n_samples * 3 doubles (nullable); finish_ms: n doubles, time
+ * from launch start to the end of the tunnel's last step (nullable). */
+int cfd_batch_results(cfd_batch *b, double *forces, double *finish_ms, int *n_samples,
+                      int *rounds);
+/* Current populations of tunnel k, layout pop[i*N + cell]; n = 19*N. */
+int cfd_batch_get_populations(cfd_batch *b, int k, float *out, size_t n);
+/* Residency limit of the batch kernel for a workgroup size. */
+int cfd_batch_max_grid(int block, int *workgroups_per_cu, int *compute_units);
+void cfd_batch_destroy(cfd_batch *b);
+
 /* voxelizar (src/geom/voxel.js:376-433) on the GPU: band rasterization with
  * the exact triangle/box overlap test (61-98) and point-triangle distance
  * (105-151) in float64 without contraction, 26-neighbour seal dilation
