@@ -93,6 +93,7 @@ def _load() -> C.CDLL:
                                     C.POINTER(C.c_double)]),
         "cfd_batch_results": (C.c_int, [P, P, P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "cfd_batch_get_populations": (C.c_int, [P, C.c_int, P, C.c_size_t]),
+        "cfd_batch_set_populations": (C.c_int, [P, C.c_int, P, C.c_size_t]),
         "cfd_batch_max_grid": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "cfd_batch_destroy": (None, [P]),
         "cfd_last_error": (C.c_char_p, []),
@@ -115,7 +116,8 @@ EXPORTS = (
     "cfd_mass", "cfd_synchronize", "cfd_bytes", "cfd_destroy", "cfd_voxelize",
     "cfd_copy_bandwidth", "cfd_last_error", "cfd_version",
     "cfd_batch_create", "cfd_batch_reset", "cfd_batch_run", "cfd_batch_results",
-    "cfd_batch_get_populations", "cfd_batch_max_grid", "cfd_batch_destroy",
+    "cfd_batch_get_populations", "cfd_batch_set_populations", "cfd_batch_max_grid",
+    "cfd_batch_destroy",
 )
 
 

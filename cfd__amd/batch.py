@@ -87,6 +87,10 @@ class LoteMI355X:
         check(lib.cfd_batch_get_populations(self._h, k, ptr(a), a.size))
         return a.reshape(Q, N)
 
+    def definirPopulacoes(self, k: int, pops: np.ndarray) -> None:
+        a = np.ascontiguousarray(pops, dtype=np.float32).reshape(-1)
+        check(lib.cfd_batch_set_populations(self._h, k, ptr(a), a.size))
+
     def autoajustar(self, passos: int = 40, candidatos: list[dict] | None = None) -> dict:
         """Times `passos` steps per candidate (state restored afterwards) and
         keeps the fastest; returns {'melhor': ..., 'medidas': [...]}."""

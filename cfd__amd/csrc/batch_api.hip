@@ -55,6 +55,7 @@ struct cfd_batch {
   std::vector<uint32_t> cells;
   BatchTunnel* dev = nullptr;
   hipStream_t stream = nullptr;
+  int cur = 0;  // every tunnel's current state is in pop[cur]
   // last run
   int steps = 0, sample_every = 0, max_samples = 0, rounds = 0;
   std::vector<uint32_t> finish_round;
@@ -199,7 +200,7 @@ int cfd_batch_reset(cfd_batch* b) {
     BCK(launch_init(Store::F32, h.pop[0], h.tipo8, h.g, h.p, b->stream));
   }
   BCK(hipStreamSynchronize(b->stream));
-  b->steps = 0;  // the initial state is in pop[0]
+  b->cur = 0;
   return CFD_OK;
 }
 
@@ -255,7 +256,8 @@ int cfd_batch_run(cfd_batch* b, int steps, int sample_every, const cfd_batch_tun
   if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
   if (e == hipSuccess)
     e = launch_batch_run(b->dev, dph, (int)phases.size(), den, t.tile_cells, sample_every,
-                         b->max_samples, b->samples, b->clocks, t.block, grid, b->stream);
+                         b->max_samples, (uint32_t)b->cur, b->samples, b->clocks, t.block, grid,
+                         b->stream);
   if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float ms = 0.f;
@@ -269,6 +271,7 @@ int cfd_batch_run(cfd_batch* b, int steps, int sample_every, const cfd_batch_tun
     return bfail(CFD_E_HIP, "cfd_batch_run: %s", hipGetErrorString(e));
   }
   b->tuning = t;
+  b->cur ^= steps & 1;
   if (ms_out) *ms_out = ms;
   return CFD_OK;
 }
@@ -305,8 +308,7 @@ int cfd_batch_get_populations(cfd_batch* b, int k, float* out, size_t n) {
   if (n != (size_t)Q * h.g.N)
     return bfail(CFD_E_ARG, "cfd_batch_get_populations: %zu values, expected %zu", n,
                  (size_t)Q * h.g.N);
-  // after `steps` steps the current state is in pop[steps & 1]
-  const void* src = h.pop[b->steps & 1];
+  const void* src = h.pop[b->cur];
   float* tmp = nullptr;
   BCK(hipMalloc(&tmp, n * sizeof(float)));
   hipError_t e = launch_store_to_f32(Store::F32, src, tmp, h.g, b->stream);
@@ -315,6 +317,25 @@ int cfd_batch_get_populations(cfd_batch* b, int k, float* out, size_t n) {
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return bfail(CFD_E_HIP, "cfd_batch_get_populations: %s",
+                                    hipGetErrorString(e));
+  return CFD_OK;
+}
+
+int cfd_batch_set_populations(cfd_batch* b, int k, const float* in, size_t n) {
+  cfd__set_error("");
+  if (!b || !in || k < 0 || k >= b->n)
+    return bfail(CFD_E_ARG, "cfd_batch_set_populations: bad arguments");
+  const BatchTunnel& h = b->host[k];
+  if (n != (size_t)Q * h.g.N)
+    return bfail(CFD_E_ARG, "cfd_batch_set_populations: %zu values, expected %zu", n,
+                 (size_t)Q * h.g.N);
+  float* tmp = nullptr;
+  BCK(hipMalloc(&tmp, n * sizeof(float)));
+  hipError_t e = hipMemcpyAsync(tmp, in, n * sizeof(float), hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = launch_f32_to_store(Store::F32, tmp, h.pop[b->cur], h.g, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return bfail(CFD_E_HIP, "cfd_batch_set_populations: %s",
                                     hipGetErrorString(e));
   return CFD_OK;
 }

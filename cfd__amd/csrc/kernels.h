@@ -90,7 +90,7 @@ hipError_t launch_copy(const float4* in, float4* out, size_t n4, hipStream_t st)
 struct BatchTunnel {       // one tunnel: geometry, uniforms, buffers
   Geom g;
   StepParams p;
-  void* pop[2];            // float32, blocked layout; step s reads pop[s & 1]
+  void* pop[2];            // float32, blocked layout; step s reads pop[(s + parity0) & 1]
   const uint8_t* tipo8;
 };
 struct BatchPhase {        // rounds [r0, r1) with a fixed admitted set
@@ -108,7 +108,7 @@ struct BatchEntry {        // one admitted tunnel inside a phase
 hipError_t batch_max_grid(int block, int* max_blocks_per_cu, int* cus);
 hipError_t launch_batch_run(const BatchTunnel* tun, const BatchPhase* phases, int nphases,
                             const BatchEntry* entries, int tile_cells, int sample_every,
-                            int max_samples, double* samples, unsigned long long* clocks,
-                            int block, int grid, hipStream_t st);
+                            int max_samples, uint32_t parity0, double* samples,
+                            unsigned long long* clocks, int block, int grid, hipStream_t st);
 
 }  // namespace cfd

@@ -30,10 +30,28 @@ namespace cfd {
 
 // ------------------------------------------------------------------ storage
 
+// Global address space.  Pointers that the kernels read from device memory
+// (the batch descriptors) carry no address space; accessing through a
+// GLOBAL-qualified pointer keeps their loads and stores in the global form
+// with a wave-uniform base in SGPRs instead of FLAT with per-lane 64-bit
+// addresses.
+#define GLOBAL __attribute__((address_space(1)))
+
+template <class T>
+__device__ __forceinline__ const GLOBAL T* gptr(const T* p) {
+  return (const GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ GLOBAL T* gptr(T* p) {
+  return (GLOBAL T*)p;
+}
+
 struct StoreF32 {
   using T = float;
-  static __device__ __forceinline__ float ld(const T* p) { return *p; }
-  static __device__ __forceinline__ void st(T* p, float v) { *p = v; }
+  template <class P>
+  static __device__ __forceinline__ float ld(P p) { return *p; }
+  template <class P>
+  static __device__ __forceinline__ void st(P p, float v) { *p = v; }
 };
 
 // float16 deviations g_i = f_i - w_i scaled by 2^12 (exact power of two, so
@@ -41,10 +59,12 @@ struct StoreF32 {
 // down to 1.5e-8 stay normal.  Collision still runs in float32.
 struct StoreF16 {
   using T = _Float16;
-  static __device__ __forceinline__ float ld(const T* p) {
+  template <class P>
+  static __device__ __forceinline__ float ld(P p) {
     return static_cast<float>(*p) * 0.000244140625f;
   }
-  static __device__ __forceinline__ void st(T* p, float v) { *p = static_cast<T>(v * 4096.0f); }
+  template <class P>
+  static __device__ __forceinline__ void st(P p, float v) { *p = static_cast<T>(v * 4096.0f); }
 };
 
 // ------------------------------------------------------------------ indexing
@@ -62,6 +82,14 @@ template <class T>
 __device__ __forceinline__ T* at(T* base, uint32_t idx) {
   return reinterpret_cast<T*>(reinterpret_cast<char*>(base) +
                               (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ const GLOBAL T* at(const GLOBAL T* base, uint32_t idx) {
+  return (const GLOBAL T*)((const GLOBAL char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ GLOBAL T* at(GLOBAL T* base, uint32_t idx) {
+  return (GLOBAL T*)((GLOBAL char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
 }
 
 // Address of population i of cell c.  LB = 0: SoA, slice stride g.S, a
@@ -192,9 +220,10 @@ __global__ __launch_bounds__(kBlock) void k_prep_types(const uint32_t* __restric
 // of a FLUIDO cell from a SOLIDO neighbour x + c_j re-reads g_j(x), which is
 // exactly the link term c_j * 2 (g_j + w_j) of wgsl.js:358-379.
 template <class S, bool MACROS, int LB, bool FORCE>
-__device__ __forceinline__ void step_cell(uint32_t cell, const typename S::T* __restrict__ src,
-                                          typename S::T* __restrict__ dst,
-                                          const uint8_t* __restrict__ tipo,
+__device__ __forceinline__ void step_cell(uint32_t cell,
+                                          const GLOBAL typename S::T* __restrict__ src,
+                                          GLOBAL typename S::T* __restrict__ dst,
+                                          const GLOBAL uint8_t* __restrict__ tipo,
                                           float4* __restrict__ macros, const Geom& g,
                                           const StepParams& p, double* f) {
   const Coord c = split(cell, g);
@@ -374,7 +403,8 @@ __global__ __launch_bounds__(BLOCK) void k_step(const typename S::T* __restrict_
                                                 StepParams p, uint32_t per_xcd, int remap) {
   const uint32_t cell = logical_block(per_xcd, remap) * BLOCK + threadIdx.x;
   if (cell >= g.N) return;
-  step_cell<S, MACROS, LB, false>(cell, src, dst, tipo, macros, g, p, nullptr);
+  step_cell<S, MACROS, LB, false>(cell, gptr(src), gptr(dst), gptr(tipo), macros, g, p,
+                                  nullptr);
 }
 
 // Workgroup sum of 3 doubles for any BLOCK (wave64 butterfly, then the
@@ -418,7 +448,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_run(const BatchTunnel* __restri
                                                      int nphases,
                                                      const BatchEntry* __restrict__ entries,
                                                      int tile_cells, int sample_every,
-                                                     int max_samples,
+                                                     int max_samples, uint32_t parity0,
                                                      double* __restrict__ samples,
                                                      unsigned long long* __restrict__ clocks) {
   namespace cg = cooperative_groups;
@@ -437,19 +467,21 @@ __global__ __launch_bounds__(BLOCK) void k_batch_run(const BatchTunnel* __restri
         const BatchEntry E = entries[P.first + lo];
         const BatchTunnel& D = tun[E.tunnel];
         const uint32_t step = r - E.start_round;
-        const float* src = static_cast<const float*>(D.pop[step & 1]);
-        float* dst = static_cast<float*>(D.pop[(step & 1) ^ 1]);
+        const uint32_t from = (step + parity0) & 1u;
+        const GLOBAL float* src = gptr(static_cast<const float*>(D.pop[from]));
+        GLOBAL float* dst = gptr(static_cast<float*>(D.pop[from ^ 1u]));
+        const GLOBAL uint8_t* tipo = gptr(D.tipo8);
         const uint32_t c0 = (t - E.tile_prefix) * (uint32_t)tile_cells;
         const uint32_t c1 = min(c0 + (uint32_t)tile_cells, D.g.N);
         const bool sample = sample_every > 0 && ((step + 1) % (uint32_t)sample_every) == 0;
         double f[3] = {0.0, 0.0, 0.0};
         if (sample) {
           for (uint32_t cell = c0 + threadIdx.x; cell < c1; cell += BLOCK)
-            step_cell<StoreF32, false, kBlockLog2, true>(cell, src, dst, D.tipo8, nullptr, D.g,
+            step_cell<StoreF32, false, kBlockLog2, true>(cell, src, dst, tipo, nullptr, D.g,
                                                          D.p, f);
         } else {
           for (uint32_t cell = c0 + threadIdx.x; cell < c1; cell += BLOCK)
-            step_cell<StoreF32, false, kBlockLog2, false>(cell, src, dst, D.tipo8, nullptr,
+            step_cell<StoreF32, false, kBlockLog2, false>(cell, src, dst, tipo, nullptr,
                                                           D.g, D.p, f);
         }
         if (sample) {  // uniform over the workgroup
@@ -841,11 +873,12 @@ hipError_t batch_max_grid(int block, int* max_blocks_per_cu, int* cus) {
 
 hipError_t launch_batch_run(const BatchTunnel* tun, const BatchPhase* phases, int nphases,
                             const BatchEntry* entries, int tile_cells, int sample_every,
-                            int max_samples, double* samples, unsigned long long* clocks,
-                            int block, int grid, hipStream_t st) {
+                            int max_samples, uint32_t parity0, double* samples,
+                            unsigned long long* clocks, int block, int grid, hipStream_t st) {
   void* args[] = {(void*)&tun,          (void*)&phases,      (void*)&nphases,
                   (void*)&entries,      (void*)&tile_cells,  (void*)&sample_every,
-                  (void*)&max_samples,  (void*)&samples,     (void*)&clocks};
+                  (void*)&max_samples,  (void*)&parity0,     (void*)&samples,
+                  (void*)&clocks};
   if (block == 512)
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_batch_run<512>),
                                       dim3(grid), dim3(512), args, 0, st);

@@ -1,0 +1,196 @@
+"""Batch workload: many small sphere / cylinder tunnels in one launch.
+
+gerar_casos(n, seed) draws n independent tunnels: a sphere (D 6-24 cells) or
+a y-spanning cylinder (D 8-20 cells, the reference's quasi-infinite cylinder
+of tests/validacao.html), Re uniform in [20, 200], and a grid scaled from
+64x32x32 up to 192x96x96 with a cross-section of at least 4 D.  Every tunnel
+is the reference's validation tunnel (inlet at x = 0, outlet at nx - 1,
+free-slip side walls, mirror floor, sponge) at u = 0.08 with nu = u D / Re and
+lesCs = 0, float32.  Cylinders that shed (Re > 47) start from a transverse
+velocity kick behind the body, the role of the inlet yaw in
+validacao.html:214-226.
+
+rodar_lote(casos, passos) builds one LoteMI355X, autotunes the launch on a
+short run, restores the initial state, runs every tunnel `passos` steps in ONE
+launch with the force sampled every `amostra` steps, and reports the
+aggregate MLUPS, per-case Cd (Maskell-corrected as validacao.html does) and
+St (zero crossings, forces.js:151-177) against the correlations of
+validacao.js, and the p50 / p99 time to solution.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import time
+
+import numpy as np
+
+from . import lattice
+from .forces import maskell, strouhal
+from .tunel import areaFrontalCorpo, corpoAnalitico, montarTipos
+from .validacao import cilindroCd, cilindroStrouhal, classificar, esferaCd
+
+U_LB = 0.08
+
+
+def gerar_casos(n: int = 256, seed: int = 2024, u: float = U_LB) -> list[dict]:
+    rng = np.random.default_rng(seed)
+    casos = []
+    for k in range(n):
+        forma = "esfera" if rng.random() < 0.5 else "cilindro"
+        D = float(rng.integers(6, 25)) if forma == "esfera" else float(rng.integers(8, 21))
+        re = float(rng.uniform(20.0, 200.0))
+        smin = max(1.0, 4.0 * D / 32.0)
+        s = float(rng.uniform(smin, 3.0))
+        ny = nz = int(round(32 * s))
+        nx = 2 * ny
+        grade = {"nx": nx, "ny": ny, "nz": nz}
+        nu = u * D / re
+        omega = 1.0 / (3.0 * nu + 0.5)
+        # a fraction of a cell off the axis: an asymmetric voxel shape
+        cx, cy, cz = nx * 0.3, ny * 0.5 + 0.25, nz * 0.5 + 0.37
+        casos.append({"id": k, "forma": forma, "D": D, "re": re, "grade": grade, "u": u,
+                      "omega": omega, "centro": (cx, cy, cz)})
+    return casos
+
+
+def montar(caso: dict) -> dict:
+    """Cell types, sponge and frontal area of one case (validacao.html:186-206)."""
+    g = caso["grade"]
+    cx, cy, cz = caso["centro"]
+    corpo = corpoAnalitico(g, {"tipo": caso["forma"], "cx": cx, "cy": cy, "cz": cz,
+                               "raio": caso["D"] / 2})
+    tipo, esponja, _ = montarTipos(g, corpo, esteira=False)
+    nx, ny = g["nx"], g["ny"]
+    tipo.reshape(g["nz"], ny, nx)[0, :, 1:nx - 1] = 6   # mirror floor: body in free air
+    return {"tipo": tipo, "esponja": esponja, "area": areaFrontalCorpo(corpo, g)}
+
+
+def campo_inicial(caso: dict, tipo: np.ndarray) -> np.ndarray:
+    """Free stream with a transverse (z) kick of 0.2 u exp(-r^2 / D^2) centred
+    one diameter downstream of a cylinder; solids at rest."""
+    g = caso["grade"]
+    nx, ny, nz = g["nx"], g["ny"], g["nz"]
+    cx, _, cz = caso["centro"]
+    D, u = caso["D"], caso["u"]
+    out = np.empty((19, nz, ny, nx), dtype=np.float32)
+    out[:] = lattice.equilibrium(0.0, [u, 0.0, 0.0]).astype(np.float32)[:, None, None, None]
+    # the kick is below 1e-4 u beyond 3 D
+    x0, x1 = max(0, int(cx - 2 * D)), min(nx, int(cx + 4 * D) + 1)
+    z0, z1 = max(0, int(cz - 3 * D)), min(nz, int(cz + 3 * D) + 1)
+    z, x = np.meshgrid(np.arange(z0, z1, dtype=np.float64) + 0.5,
+                       np.arange(x0, x1, dtype=np.float64) + 0.5, indexing="ij")
+    uz = 0.2 * u * np.exp(-((x - cx - D) ** 2 + (z - cz) ** 2) / (D * D))
+    feq = lattice.equilibrium(np.zeros_like(uz), [np.full_like(uz, u), np.zeros_like(uz), uz])
+    out[:, z0:z1, :, x0:x1] = feq.astype(np.float32)[:, :, None, :]
+    solido = tipo.reshape(nz, ny, nx) == 1
+    rest = lattice.equilibrium(0.0, [0.0, 0.0, 0.0]).astype(np.float32)
+    out[:, solido] = rest[:, None]
+    return out.reshape(19, -1)
+
+
+def _tuneis(casos, montados):
+    out = []
+    for c, m in zip(casos, montados):
+        out.append({"grade": c["grade"], "tipo": m["tipo"],
+                    "cfg": {"omegaPlus": c["omega"], "lesCs": 0.0, "inletU": [c["u"], 0.0, 0.0],
+                            "esponja": m["esponja"]}})
+    return out
+
+
+def _perturbar(lote, casos, montados):
+    for k, (c, m) in enumerate(zip(casos, montados)):
+        if c["forma"] == "cilindro" and c["re"] > 47:
+            lote.definirPopulacoes(k, campo_inicial(c, m["tipo"]))
+
+
+def rodar_lote(casos: list[dict], passos: int = 5000, amostra: int = 10,
+               autoajuste_passos: int = 40, candidatos: list[dict] | None = None,
+               ajuste: dict | None = None, log=print) -> dict:
+    from .batch import LoteMI355X
+
+    t0 = time.perf_counter()
+    montados = [montar(c) for c in casos]
+    lote = LoteMI355X(_tuneis(casos, montados))
+    try:
+        log(f"[lote] {len(casos)} tunnels, {int(lote.celulas.sum())} cells, set-up "
+            f"{time.perf_counter() - t0:.1f} s")
+        tun = None
+        if ajuste is None:
+            tun = lote.autoajustar(passos=autoajuste_passos, candidatos=candidatos)
+            ajuste = tun["melhor"]
+            log(f"[lote] autotuner: {ajuste}")
+        else:
+            lote.reiniciar()
+        _perturbar(lote, casos, montados)
+        ms = lote.rodar(passos, amostra, ajuste)
+        res = lote.resultados()
+    finally:
+        lote.destruir()
+    cells = np.array([c["grade"]["nx"] * c["grade"]["ny"] * c["grade"]["nz"] for c in casos],
+                     dtype=np.float64)
+    mlups = cells.sum() * passos / (ms * 1e-3) / 1e6
+    fim = res["fim_ms"]
+    f = res["forcas"]               # (n, samples, 3)
+    nmed = max(1, int(f.shape[1] * 0.4))   # the last 40 % of the run
+    casos_out = []
+    for k, c in enumerate(casos):
+        q = 0.5 * c["u"] * c["u"] * max(montados[k]["area"], 1)
+        g = c["grade"]
+        bloq = montados[k]["area"] / (g["ny"] * g["nz"])
+        theta = 1.0 if c["forma"] == "cilindro" else 2.5
+        cd_raw = f[k, -nmed:, 0] / q
+        cd = float(np.mean([maskell(v, bloq, theta) for v in cd_raw]))
+        ref = esferaCd(c["re"]) if c["forma"] == "esfera" else cilindroCd(c["re"])
+        row = {"id": c["id"], "forma": c["forma"], "D": c["D"], "re": round(c["re"], 3),
+               "grade": [g["nx"], g["ny"], g["nz"]], "cd": cd, "cdRef": ref,
+               "cdErroPct": classificar(cd, ref)["pct"], "ttsMs": float(fim[k]),
+               "finito": bool(np.isfinite(f[k]).all())}
+        if c["forma"] == "cilindro":
+            cl = list(f[k, -nmed:, 2] / q)
+            st = strouhal(cl, amostra, c["D"], c["u"])
+            row["st"] = st["st"]
+            row["stRef"] = cilindroStrouhal(c["re"])
+            if "motivo" in st:
+                row["stMotivo"] = st["motivo"]
+        casos_out.append(row)
+    cd_err = np.array([r["cdErroPct"] for r in casos_out])
+    st_rows = [r for r in casos_out if r.get("st") is not None and np.isfinite(r["st"])
+               and r["stRef"] > 0]
+    st_err = np.array([abs(r["st"] - r["stRef"]) / r["stRef"] * 100 for r in st_rows])
+    resumo = {
+        "tunnels": len(casos), "cells": int(cells.sum()), "steps": passos,
+        "launch_ms": ms, "aggregate_mlups": mlups, "tuning": ajuste,
+        "tts_ms_p50": float(np.percentile(fim, 50)), "tts_ms_p99": float(np.percentile(fim, 99)),
+        "tts_ms_max": float(fim.max()), "rounds": res["rodadas"],
+        "cd_err_pct_median": float(np.median(cd_err)),
+        "cd_err_pct_p90": float(np.percentile(cd_err, 90)),
+        "cd_within_15pct": int((cd_err <= 15).sum()),
+        "st_cases": len(st_rows),
+        "st_err_pct_median": float(np.median(st_err)) if st_rows else None,
+        "all_finite": all(r["finito"] for r in casos_out),
+        "autotune": tun["medidas"] if tun else None,
+    }
+    return {"resumo": resumo, "casos": casos_out}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--casos", type=int, default=256)
+    ap.add_argument("--passos", type=int, default=5000)
+    ap.add_argument("--amostra", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--saida", default=None)
+    a = ap.parse_args(argv)
+    out = rodar_lote(gerar_casos(a.casos, a.seed), a.passos, a.amostra)
+    if a.saida:
+        with open(a.saida, "w") as fh:
+            json.dump(out, fh, indent=1)
+    r = dict(out["resumo"])
+    r.pop("autotune", None)
+    print(json.dumps(r))
+
+
+if __name__ == "__main__":
+    main()

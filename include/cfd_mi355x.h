@@ -200,8 +200,12 @@ int cfd_batch_run(cfd_batch *b, int steps, int sample_every, const cfd_batch_tun
  * from launch start to the end of the tunnel's last step (nullable). */
 int cfd_batch_results(cfd_batch *b, double *forces, double *finish_ms, int *n_samples,
                       int *rounds);
-/* Current populations of tunnel k, layout pop[i*N + cell]; n = 19*N. */
+/* Current populations of tunnel k, layout pop[i*N + cell]; n = 19*N.  Runs
+ * continue from the current state (cfd_batch_reset returns to the
+ * free-stream equilibrium). */
 int cfd_batch_get_populations(cfd_batch *b, int k, float *out, size_t n);
+/* Replace the current state of tunnel k (e.g. a perturbed initial field). */
+int cfd_batch_set_populations(cfd_batch *b, int k, const float *in, size_t n);
 /* Residency limit of the batch kernel for a workgroup size. */
 int cfd_batch_max_grid(int block, int *workgroups_per_cu, int *compute_units);
 void cfd_batch_destroy(cfd_batch *b);

@@ -100,3 +100,35 @@ def test_batch_admission_orders_completion_and_autotuner_restores_state():
     order = np.argsort(cells, kind="stable")
     # smallest first: completion times are non-decreasing in admission order
     assert all(fin[order[i]] <= fin[order[i + 1]] + 1e-9 for i in range(len(order) - 1))
+
+
+def test_batch_runs_continue_and_accept_a_state():
+    """Two launches of 3 and 4 steps equal one tunnel stepped 7 times from the
+    same uploaded (perturbed) state, whatever the parity between launches."""
+    from cfd__amd.batch import LoteMI355X
+    from cfd__amd.solver import SolverMI355X
+    c = CASES[1]
+    g0 = _init_pops(c)
+    rng = np.random.default_rng(5)
+    g0 = (g0 * (1 + 0.01 * rng.standard_normal(g0.shape))).astype(np.float32)
+    lote = LoteMI355X([CASES[0], c])
+    try:
+        lote.definirPopulacoes(1, g0)
+        assert np.array_equal(lote.populacoes(1), g0)
+        lote.rodar(3, 0, {"block": 512, "tile_cells": 2048})
+        lote.rodar(4, 0, {"block": 256, "tile_cells": 1024})
+        got = lote.populacoes(1)
+    finally:
+        lote.destruir()
+    s = SolverMI355X.criar(c["grade"])
+    try:
+        s.definirTipos(c["tipo"])
+        s.configurar(**c["cfg"])
+        s.inicializar()
+        s.definirPopulacoes(g0)
+        s.passo(7)
+        ref = s.lerPopulacoes()
+    finally:
+        s.destruir()
+    fluid = ~np.isin(c["tipo"], [1, 4, 7])   # wall cells are never written
+    assert np.array_equal(got[:, fluid], ref[:, fluid])
