@@ -11,7 +11,6 @@ tile shapes and launch sizes and keeps the fastest.
 from __future__ import annotations
 
 import ctypes as C
-import itertools
 
 import numpy as np
 
@@ -95,14 +94,16 @@ class LoteMI355X:
         """Times `passos` steps per candidate (state restored afterwards) and
         keeps the fastest; returns {'melhor': ..., 'medidas': [...]}."""
         if candidatos is None:
+            # tile shape x workgroup size x admission cap; the cap is given
+            # in cells (mult x 4096 per resident workgroup) so that every
+            # tile size admits the same amount of work per round
             candidatos = []
-            for block, tile in itertools.product((256, 512), (512, 1024, 2048, 4096)):
+            for block in (256, 512):
                 per, cus = self.limite_residencia(block)
-                for wg in sorted({per, max(1, per // 2)}):
-                    grid = wg * cus
-                    for mult in (4, 16):
-                        candidatos.append({"block": block, "tile_cells": tile, "wg_per_cu": wg,
-                                           "max_active_tiles": grid * mult})
+                for tile in (512, 1024, 2048, 4096):
+                    for mult in (8, 32):
+                        candidatos.append({"block": block, "tile_cells": tile, "wg_per_cu": per,
+                                           "max_active_tiles": per * cus * mult * 4096 // tile})
         medidas = []
         for c in candidatos:
             ms = self.rodar(passos, 0, c)

@@ -85,7 +85,8 @@ void free_batch(cfd_batch* b) {
 // Admission schedule: phases of rounds with a fixed admitted set.
 void build_schedule(const cfd_batch* b, int steps, uint32_t tile_cells, uint32_t cap,
                     std::vector<BatchPhase>* phases, std::vector<BatchEntry>* entries,
-                    std::vector<uint32_t>* finish, uint32_t* rounds) {
+                    std::vector<uint32_t>* table, std::vector<uint32_t>* finish,
+                    uint32_t* rounds) {
   std::vector<int> order(b->n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(),
@@ -114,9 +115,12 @@ void build_schedule(const cfd_batch* b, int steps, uint32_t tile_cells, uint32_t
     ph.first = (uint32_t)entries->size();
     ph.n_active = (uint32_t)active.size();
     uint32_t pre = 0;
-    for (auto& a : active) {
+    ph.table = (uint32_t)table->size();
+    for (size_t e = 0; e < active.size(); ++e) {
+      const auto& a = active[e];
       entries->push_back({(uint32_t)a.first, a.second, pre, 0u});
       pre += tiles(a.first);
+      table->resize(ph.table + pre, (uint32_t)e);
     }
     ph.total_tiles = pre;
     phases->push_back(ph);
@@ -223,9 +227,10 @@ int cfd_batch_run(cfd_batch* b, int steps, int sample_every, const cfd_batch_tun
 
   std::vector<BatchPhase> phases;
   std::vector<BatchEntry> entries;
+  std::vector<uint32_t> table;
   uint32_t rounds = 0;
-  build_schedule(b, steps, (uint32_t)t.tile_cells, cap, &phases, &entries, &b->finish_round,
-                 &rounds);
+  build_schedule(b, steps, (uint32_t)t.tile_cells, cap, &phases, &entries, &table,
+                 &b->finish_round, &rounds);
   b->steps = steps;
   b->sample_every = sample_every;
   b->max_samples = sample_every > 0 ? steps / sample_every : 0;
@@ -233,8 +238,12 @@ int cfd_batch_run(cfd_batch* b, int steps, int sample_every, const cfd_batch_tun
 
   BatchPhase* dph = nullptr;
   BatchEntry* den = nullptr;
+  uint32_t* dtab = nullptr;
+  uint32_t* dctr = nullptr;
   BCK(hipMalloc(&dph, sizeof(BatchPhase) * phases.size()));
   hipError_t e = hipMalloc(&den, sizeof(BatchEntry) * entries.size());
+  if (e == hipSuccess) e = hipMalloc(&dtab, sizeof(uint32_t) * table.size());
+  if (e == hipSuccess) e = hipMalloc(&dctr, sizeof(uint32_t) * rounds);
   if (b->samples) (void)hipFree(b->samples);
   if (b->clocks) (void)hipFree(b->clocks);
   b->samples = nullptr;
@@ -249,23 +258,38 @@ int cfd_batch_run(cfd_batch* b, int steps, int sample_every, const cfd_batch_tun
   if (e == hipSuccess)
     e = hipMemcpyAsync(den, entries.data(), sizeof(BatchEntry) * entries.size(),
                        hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dtab, table.data(), sizeof(uint32_t) * table.size(),
+                       hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(dctr, 0, sizeof(uint32_t) * rounds, b->stream);
   if (e == hipSuccess) e = hipMemsetAsync(b->samples, 0, ns * sizeof(double), b->stream);
+  BatchLaunch L{};
+  L.phases = dph;
+  L.entries = den;
+  L.tile_entry = dtab;
+  L.counters = dctr;
+  L.samples = b->samples;
+  L.clocks = b->clocks;
+  L.nphases = (int)phases.size();
+  L.tile_cells = t.tile_cells;
+  L.sample_every = sample_every;
+  L.max_samples = b->max_samples;
+  L.parity0 = (uint32_t)b->cur;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
-  if (e == hipSuccess)
-    e = launch_batch_run(b->dev, dph, (int)phases.size(), den, t.tile_cells, sample_every,
-                         b->max_samples, (uint32_t)b->cur, b->samples, b->clocks, t.block, grid,
-                         b->stream);
+  if (e == hipSuccess) e = launch_batch_run(b->dev, L, t.block, grid, b->stream);
   if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float ms = 0.f;
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  (void)hipFree(dph);
+  if (dph) (void)hipFree(dph);
   if (den) (void)hipFree(den);
+  if (dtab) (void)hipFree(dtab);
+  if (dctr) (void)hipFree(dctr);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return bfail(CFD_E_HIP, "cfd_batch_run: %s", hipGetErrorString(e));

@@ -98,6 +98,7 @@ struct BatchPhase {        // rounds [r0, r1) with a fixed admitted set
   uint32_t first;          // index of its first BatchEntry
   uint32_t n_active;
   uint32_t total_tiles;
+  uint32_t table;          // offset of its tile -> entry table
 };
 struct BatchEntry {        // one admitted tunnel inside a phase
   uint32_t tunnel;
@@ -105,10 +106,18 @@ struct BatchEntry {        // one admitted tunnel inside a phase
   uint32_t tile_prefix;    // first tile index of this tunnel in the phase
   uint32_t pad;
 };
+struct BatchLaunch {       // everything k_batch_run reads besides the tunnels
+  const BatchPhase* phases;
+  const BatchEntry* entries;
+  const uint32_t* tile_entry;  // per phase: tile -> index of its entry in the phase
+  uint32_t* counters;          // one work counter per round, zero at launch
+  double* samples;             // [tunnel][sample][3]
+  unsigned long long* clocks;  // [0] launch start, [1 + r] end of round r
+  int nphases, tile_cells, sample_every, max_samples;
+  uint32_t parity0;            // every tunnel's current state is in pop[parity0]
+};
 hipError_t batch_max_grid(int block, int* max_blocks_per_cu, int* cus);
-hipError_t launch_batch_run(const BatchTunnel* tun, const BatchPhase* phases, int nphases,
-                            const BatchEntry* entries, int tile_cells, int sample_every,
-                            int max_samples, uint32_t parity0, double* samples,
-                            unsigned long long* clocks, int block, int grid, hipStream_t st);
+hipError_t launch_batch_run(const BatchTunnel* tun, const BatchLaunch& L, int block, int grid,
+                            hipStream_t st);
 
 }  // namespace cfd

@@ -435,71 +435,79 @@ __device__ __forceinline__ void block_sum_into(double (&v)[3], double (&out)[3])
 // Many independent tunnels in ONE cooperative launch.  The host builds a
 // schedule of rounds: in round r every admitted tunnel advances one step,
 // its cells split into tiles of `tile_cells` consecutive cells; the tiles of
-// all admitted tunnels form one flat list that the workgroups stride over,
-// then the whole grid synchronizes (all workgroups are co-resident: the
-// runtime refuses a cooperative launch that does not fit).  Rounds with the
-// same admitted set form a phase.  A tunnel's parity (A or B as source) is
-// its own step count, so tunnels admitted at different rounds coexist.
+// all admitted tunnels form one flat list, then the whole grid synchronizes
+// (all workgroups are co-resident: the runtime refuses a cooperative launch
+// that does not fit).  Rounds with the same admitted set form a phase.  A
+// tunnel's parity (A or B as source) is its own step count, so tunnels
+// admitted at different rounds coexist.
+//
+// Workgroups take tiles from a per-round counter, in list order, the way the
+// hardware dispatcher hands out workgroups of an ordinary launch: the tiles
+// in flight stay a contiguous window of the list, so the z-neighbour planes
+// a tile pulls from are being read by its neighbours in the window at the
+// same time and come from cache, not again from HBM.  (A static stride lets
+// workgroups drift apart over a round and those re-reads miss.)  Each
+// workgroup requests its next tile before stepping the current one, so the
+// counter's latency is hidden behind the tile.
+//
 // Sampled steps also accumulate the body force (float64 per workgroup, one
 // atomic add per workgroup and component into the tunnel's sample slot).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_batch_run(const BatchTunnel* __restrict__ tun,
-                                                     const BatchPhase* __restrict__ phases,
-                                                     int nphases,
-                                                     const BatchEntry* __restrict__ entries,
-                                                     int tile_cells, int sample_every,
-                                                     int max_samples, uint32_t parity0,
-                                                     double* __restrict__ samples,
-                                                     unsigned long long* __restrict__ clocks) {
+                                                     const BatchLaunch L) {
   namespace cg = cooperative_groups;
   cg::grid_group grid = cg::this_grid();
-  if (blockIdx.x == 0 && threadIdx.x == 0) clocks[0] = wall_clock64();
-  for (int ph = 0; ph < nphases; ++ph) {
-    const BatchPhase P = phases[ph];
+  __shared__ uint32_t s_tile[2];
+  if (blockIdx.x == 0 && threadIdx.x == 0) L.clocks[0] = wall_clock64();
+  for (int ph = 0; ph < L.nphases; ++ph) {
+    const BatchPhase P = L.phases[ph];
     for (uint32_t r = P.r0; r < P.r1; ++r) {
-      for (uint32_t t = blockIdx.x; t < P.total_tiles; t += gridDim.x) {
-        // the admitted tunnel owning tile t (tile_prefix ascending)
-        uint32_t lo = 0, hi = P.n_active;
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (entries[P.first + mid].tile_prefix <= t) lo = mid; else hi = mid;
-        }
-        const BatchEntry E = entries[P.first + lo];
+      uint32_t* ctr = L.counters + r;
+      if (threadIdx.x == 0) s_tile[0] = atomicAdd(ctr, 1u);
+      __syncthreads();
+      uint32_t t = s_tile[0];
+      for (uint32_t it = 1; t < P.total_tiles; ++it) {
+        uint32_t next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(ctr, 1u);  // consumed after the tile
+        const BatchEntry E = L.entries[P.first + L.tile_entry[P.table + t]];
         const BatchTunnel& D = tun[E.tunnel];
         const uint32_t step = r - E.start_round;
-        const uint32_t from = (step + parity0) & 1u;
+        const uint32_t from = (step + L.parity0) & 1u;
         const GLOBAL float* src = gptr(static_cast<const float*>(D.pop[from]));
         GLOBAL float* dst = gptr(static_cast<float*>(D.pop[from ^ 1u]));
         const GLOBAL uint8_t* tipo = gptr(D.tipo8);
-        const uint32_t c0 = (t - E.tile_prefix) * (uint32_t)tile_cells;
-        const uint32_t c1 = min(c0 + (uint32_t)tile_cells, D.g.N);
-        const bool sample = sample_every > 0 && ((step + 1) % (uint32_t)sample_every) == 0;
-        double f[3] = {0.0, 0.0, 0.0};
+        const uint32_t c0 = (t - E.tile_prefix) * (uint32_t)L.tile_cells;
+        const uint32_t c1 = min(c0 + (uint32_t)L.tile_cells, D.g.N);
+        const bool sample =
+            L.sample_every > 0 && ((step + 1) % (uint32_t)L.sample_every) == 0;
         if (sample) {
+          double f[3] = {0.0, 0.0, 0.0};
           for (uint32_t cell = c0 + threadIdx.x; cell < c1; cell += BLOCK)
             step_cell<StoreF32, false, kBlockLog2, true>(cell, src, dst, tipo, nullptr, D.g,
                                                          D.p, f);
-        } else {
-          for (uint32_t cell = c0 + threadIdx.x; cell < c1; cell += BLOCK)
-            step_cell<StoreF32, false, kBlockLog2, false>(cell, src, dst, tipo, nullptr,
-                                                          D.g, D.p, f);
-        }
-        if (sample) {  // uniform over the workgroup
           double part[3];
-          block_sum_into<BLOCK>(f, part);
+          block_sum_into<BLOCK>(f, part);  // uniform branch: the whole workgroup
           if (threadIdx.x == 0) {
-            const int k = (int)((step + 1) / (uint32_t)sample_every) - 1;
-            if (k < max_samples) {
-              double* slot = samples + ((size_t)E.tunnel * max_samples + k) * 3;
+            const int k = (int)((step + 1) / (uint32_t)L.sample_every) - 1;
+            if (k < L.max_samples) {
+              double* slot = L.samples + ((size_t)E.tunnel * L.max_samples + k) * 3;
               atomicAdd(slot + 0, part[0]);
               atomicAdd(slot + 1, part[1]);
               atomicAdd(slot + 2, part[2]);
             }
           }
+        } else {
+          for (uint32_t cell = c0 + threadIdx.x; cell < c1; cell += BLOCK)
+            step_cell<StoreF32, false, kBlockLog2, false>(cell, src, dst, tipo, nullptr,
+                                                          D.g, D.p, nullptr);
         }
+        // ping-pong slot: a wave still reading the previous slot is not raced
+        if (threadIdx.x == 0) s_tile[it & 1] = next;
+        __syncthreads();
+        t = s_tile[it & 1];
       }
       grid.sync();
-      if (blockIdx.x == 0 && threadIdx.x == 0) clocks[1 + r] = wall_clock64();
+      if (blockIdx.x == 0 && threadIdx.x == 0) L.clocks[1 + r] = wall_clock64();
     }
   }
 }
@@ -871,14 +879,10 @@ hipError_t batch_max_grid(int block, int* max_blocks_per_cu, int* cus) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(max_blocks_per_cu, k_batch_run<256>, 256, 0);
 }
 
-hipError_t launch_batch_run(const BatchTunnel* tun, const BatchPhase* phases, int nphases,
-                            const BatchEntry* entries, int tile_cells, int sample_every,
-                            int max_samples, uint32_t parity0, double* samples,
-                            unsigned long long* clocks, int block, int grid, hipStream_t st) {
-  void* args[] = {(void*)&tun,          (void*)&phases,      (void*)&nphases,
-                  (void*)&entries,      (void*)&tile_cells,  (void*)&sample_every,
-                  (void*)&max_samples,  (void*)&parity0,     (void*)&samples,
-                  (void*)&clocks};
+hipError_t launch_batch_run(const BatchTunnel* tun, const BatchLaunch& L, int block, int grid,
+                            hipStream_t st) {
+  BatchLaunch l = L;
+  void* args[] = {(void*)&tun, (void*)&l};
   if (block == 512)
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_batch_run<512>),
                                       dim3(grid), dim3(512), args, 0, st);
