@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--write", type=Path, required=True)
     ap.add_argument("--kernel", default="k_step")
     ap.add_argument("--cells", type=int, required=True)
+    ap.add_argument("--steps", type=int, default=1,
+                    help="steps per launch (a batch launch runs many)")
     ap.add_argument("--bytes-per-cell", type=int, default=152)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--command", required=True)
@@ -77,7 +79,7 @@ def main():
         return best
     fk, wk = same(fetch), same(write)
     traffic = (fk * ff + wk * fw) * 1024
-    alg = a.bytes_per_cell * a.cells
+    alg = a.bytes_per_cell * a.cells * a.steps
     avg_s = main_row[2] * 1e-9
     lines = [
         f"# {a.title or a.kernel}",
@@ -89,7 +91,8 @@ def main():
         f"rocprofv3 --pmc FETCH_SIZE -d <fetch> -o run --output-format csv -- {a.command}",
         f"rocprofv3 --pmc WRITE_SIZE -d <write> -o run --output-format csv -- {a.command}",
         f"python -m cfd__amd.rocprof_summary --trace <trace> --fetch <fetch> --write <write> "
-        f"--kernel {a.kernel} --cells {a.cells} --bytes-per-cell {a.bytes_per_cell} "
+        f"--kernel {a.kernel} --cells {a.cells} --steps {a.steps} "
+        f"--bytes-per-cell {a.bytes_per_cell} "
         f"--copy-bytes {a.copy_bytes} --command '...' --out {a.out}",
         "```",
         "",
@@ -114,12 +117,14 @@ def main():
         "",
         "## Roofline",
         "",
-        f"- algorithmic_bytes_per_launch: {alg}  ({a.bytes_per_cell} B/cell x {a.cells} cells)",
+        f"- algorithmic_bytes_per_launch: {alg}  ({a.bytes_per_cell} B/cell x {a.cells} cells"
+        f" x {a.steps} steps)",
         f"- traffic_bytes_per_launch: {traffic:.0f}",
         f"- traffic / algorithmic: {traffic / alg:.4f}",
         f"- avg_kernel_us: {main_row[2] / 1e3:.2f}  (main instantiation, {main_row[1]} calls)",
         f"- achieved_algorithmic_GBps: {alg / avg_s / 1e9:.1f}",
         f"- fraction_of_8TBps_peak: {alg / avg_s / 8e12:.4f}",
+        f"- cell updates per second: {a.cells * a.steps / avg_s / 1e6:.0f} MLUPS",
         "",
     ]
     a.out.parent.mkdir(parents=True, exist_ok=True)
