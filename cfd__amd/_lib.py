@@ -48,6 +48,23 @@ class BatchTuning(C.Structure):
                 ("max_active_tiles", C.c_int32)]
 
 
+class Mlp(C.Structure):
+    """cfd_mlp: 1377 float32 in the order w1[32][8] b1 w2[32][32] b2 w3 b3."""
+    _fields_ = [("w1", C.c_float * 256), ("b1", C.c_float * 32), ("w2", C.c_float * 1024),
+                ("b2", C.c_float * 32), ("w3", C.c_float * 32), ("b3", C.c_float)]
+
+
+class Closure(C.Structure):
+    _fields_ = [("mlp", Mlp), ("in_shift", C.c_float * 8), ("in_scale", C.c_float * 8),
+                ("in_lo", C.c_float * 8), ("in_hi", C.c_float * 8), ("out_scale", C.c_float)]
+
+
+class TrainCfg(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("seed", C.c_uint64)]
+
+
 class VoxStats(C.Structure):
     _fields_ = [
         ("n_cells", C.c_int64), ("n_casca", C.c_int64), ("n_dentro", C.c_int64),
@@ -87,6 +104,13 @@ def _load() -> C.CDLL:
                                    C.c_double, C.c_int, P, P, C.POINTER(VoxStats)]),
         "cfd_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]),
+        "cfd_set_closure": (C.c_int, [P, C.POINTER(Closure)]),
+        "cfd_set_wall_distance": (C.c_int, [P, P, C.c_size_t]),
+        "cfd_closure_fallbacks": (C.c_int, [P, C.POINTER(C.c_int64), C.c_int]),
+        "cfd_mlp_train": (C.c_int, [P, P, C.c_size_t, C.POINTER(TrainCfg), C.POINTER(Mlp), P]),
+        "cfd_mlp_grad": (C.c_int, [P, P, C.c_size_t, C.POINTER(Mlp), C.POINTER(Mlp),
+                                   C.POINTER(C.c_double)]),
+        "cfd_mlp_infer": (C.c_int, [P, C.c_size_t, C.POINTER(Closure), P, P]),
         "cfd_batch_create": (C.c_int, [P, C.c_int, C.POINTER(P)]),
         "cfd_batch_reset": (C.c_int, [P]),
         "cfd_batch_run": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(BatchTuning),
@@ -115,6 +139,8 @@ EXPORTS = (
     "cfd_step", "cfd_step_timed", "cfd_set_tuning", "cfd_get_tuning", "cfd_forces", "cfd_update_macros", "cfd_read_macros",
     "cfd_mass", "cfd_synchronize", "cfd_bytes", "cfd_destroy", "cfd_voxelize",
     "cfd_copy_bandwidth", "cfd_last_error", "cfd_version",
+    "cfd_set_closure", "cfd_set_wall_distance", "cfd_closure_fallbacks", "cfd_mlp_train",
+    "cfd_mlp_grad", "cfd_mlp_infer",
     "cfd_batch_create", "cfd_batch_reset", "cfd_batch_run", "cfd_batch_results",
     "cfd_batch_get_populations", "cfd_batch_set_populations", "cfd_batch_max_grid",
     "cfd_batch_destroy",

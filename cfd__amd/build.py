@@ -41,8 +41,9 @@ HIP_FLAGS = [
 FILE_FLAGS = {
     "voxel.hip": ["-ffp-contract=off"],
     "lbm_kernels.hip": ["-ffp-contract=off"],
+    "closure.hip": ["-ffp-contract=off"],
 }
-SOURCES = ["lbm_kernels.hip", "cfd_core.hip", "batch_api.hip", "voxel.hip"]
+SOURCES = ["lbm_kernels.hip", "closure.hip", "cfd_core.hip", "batch_api.hip", "voxel.hip"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/cfd_mi355x.h"
 #include "kernels.h"
@@ -60,6 +61,10 @@ struct cfd_solver {
   double* partials = nullptr;
   double* result = nullptr;   // device, 4 doubles
   double* host_res = nullptr; // pinned, 4 doubles
+  // learned closure (off while mlp == nullptr)
+  MlpPacked* mlp = nullptr;
+  uint8_t* wall8 = nullptr;   // wall distance, 1/8 cell, saturating at 31.875
+  unsigned long long* fallbacks = nullptr;
 };
 
 namespace {
@@ -74,6 +79,9 @@ void release(cfd_solver* s) {
   if (s->partials) (void)hipFree(s->partials);
   if (s->result) (void)hipFree(s->result);
   if (s->host_res) (void)hipHostFree(s->host_res);
+  if (s->mlp) (void)hipFree(s->mlp);
+  if (s->wall8) (void)hipFree(s->wall8);
+  if (s->fallbacks) (void)hipFree(s->fallbacks);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -398,8 +406,13 @@ int cfd_get_front(cfd_solver* s, int* front) {
 static int enqueue_steps(cfd_solver* s, int n) {
   for (int k = 0; k < n; ++k) {
     const int f = s->front;
-    HIPCK(launch_step(s->store, s->pop[f], s->pop[f ^ 1], s->tipo8, s->macros, k == n - 1,
-                      s->geom, s->params, s->tuning, s->stream));
+    if (s->mlp)
+      HIPCK(launch_step_closure(s->store, s->pop[f], s->pop[f ^ 1], s->tipo8, s->wall8, s->mlp,
+                                s->macros, k == n - 1, s->geom, s->params, s->fallbacks,
+                                s->stream));
+    else
+      HIPCK(launch_step(s->store, s->pop[f], s->pop[f ^ 1], s->tipo8, s->macros, k == n - 1,
+                        s->geom, s->params, s->tuning, s->stream));
     s->front = f ^ 1;
   }
   return CFD_OK;
@@ -487,6 +500,70 @@ int cfd_bytes(cfd_solver* s, int64_t* bytes_out) {
 }
 
 void cfd_destroy(cfd_solver* s) { release(s); }
+
+static int ensure_closure_buffers(cfd_solver* s) {
+  if (!s->wall8) {
+    HIPCK(hipMalloc(&s->wall8, s->N));
+    HIPCK(hipMemsetAsync(s->wall8, 0, s->N, s->stream));
+  }
+  if (!s->fallbacks) {
+    HIPCK(hipMalloc(&s->fallbacks, sizeof(unsigned long long)));
+    HIPCK(hipMemsetAsync(s->fallbacks, 0, sizeof(unsigned long long), s->stream));
+  }
+  return CFD_OK;
+}
+
+int cfd_set_closure(cfd_solver* s, const cfd_closure* c) {
+  g_err.clear();
+  if (!s) return fail(CFD_E_ARG, "cfd_set_closure: null handle");
+  HIPCK(hipStreamSynchronize(s->stream));
+  if (!c) {
+    if (s->mlp) (void)hipFree(s->mlp);
+    s->mlp = nullptr;
+    return CFD_OK;
+  }
+  for (int k = 0; k < 8; ++k)
+    if (!std::isfinite(c->in_shift[k]) || !std::isfinite(c->in_scale[k]) ||
+        !(c->in_lo[k] <= c->in_hi[k]))
+      return fail(CFD_E_ARG, "cfd_set_closure: feature %d has a bad normalization or range", k);
+  if (!(c->out_scale >= 0.0f)) return fail(CFD_E_ARG, "cfd_set_closure: out_scale must be >= 0");
+  int rc = ensure_closure_buffers(s);
+  if (rc) return rc;
+  MlpPacked pk;
+  pack_closure(*c, &pk);
+  if (!s->mlp) HIPCK(hipMalloc(&s->mlp, sizeof(MlpPacked)));
+  HIPCK(hipMemcpy(s->mlp, &pk, sizeof pk, hipMemcpyHostToDevice));
+  return CFD_OK;
+}
+
+int cfd_set_wall_distance(cfd_solver* s, const float* d, size_t n) {
+  g_err.clear();
+  if (!s || !d) return fail(CFD_E_ARG, "cfd_set_wall_distance: null argument");
+  if (n != s->N)
+    return fail(CFD_E_ARG, "cfd_set_wall_distance: %zu values, expected %zu", n, s->N);
+  int rc = ensure_closure_buffers(s);
+  if (rc) return rc;
+  std::vector<uint8_t> q(n);
+  for (size_t k = 0; k < n; ++k) {
+    const float v = d[k] * 8.0f;
+    q[k] = !(v > 0.0f) ? 0 : (v >= 255.0f ? 255 : (uint8_t)(v + 0.5f));
+  }
+  HIPCK(hipMemcpy(s->wall8, q.data(), n, hipMemcpyHostToDevice));
+  return CFD_OK;
+}
+
+int cfd_closure_fallbacks(cfd_solver* s, int64_t* count, int reset) {
+  g_err.clear();
+  if (!s || !count) return fail(CFD_E_ARG, "cfd_closure_fallbacks: null argument");
+  *count = 0;
+  if (!s->fallbacks) return CFD_OK;
+  unsigned long long v = 0;
+  HIPCK(hipStreamSynchronize(s->stream));
+  HIPCK(hipMemcpy(&v, s->fallbacks, sizeof v, hipMemcpyDeviceToHost));
+  if (reset) HIPCK(hipMemset(s->fallbacks, 0, sizeof v));
+  *count = (int64_t)v;
+  return CFD_OK;
+}
 
 int cfd_copy_bandwidth(size_t bytes, int iters, double* gbps_out, double* ms_out) {
   g_err.clear();

@@ -1,0 +1,584 @@
+// Learned sub-grid closure: a per-cell MLP 8 -> 32 -> 32 -> 1 (SiLU, SiLU,
+// softplus) that replaces the Smagorinsky eddy viscosity inside the step.
+//
+// Inference runs on the matrix cores: bf16 operands, float32 accumulation,
+// v_mfma_f32_16x16x32_bf16.  A wave steps 64 cells; each lane holds its own
+// cell's 8 features as the B operand (k = 8 * (lane / 16) + v), and layer 1 is
+// evaluated as four 16-cell blocks by masking the A operand (W1) to one lane
+// group at a time, so no lane ever gathers another lane's features.  A 16x16
+// result tile leaves hidden unit 4 * (lane / 16) + v of cell (lane % 16) in
+// each lane, which is already the B layout of the next layer once the next
+// weight matrix has its input index permuted (done on the host).  The single
+// output lands in lanes 0-15 and one shuffle returns it to the owning lane.
+//
+// The closure step (k_step_closure) is the step of lbm_kernels.hip with the
+// eddy viscosity taken from the MLP: features are the non-equilibrium stress
+// Pi_ab / rho (6), the strain rate |S| = 3 Q_s / (2 rho tau_s) implied by the
+// Smagorinsky solution tau_s, and the wall distance; a fluid cell whose
+// normalized features leave the trained range, or whose output is not finite,
+// falls back to the Smagorinsky tau_s (counted per launch).  Every lane of a
+// wave reaches the matrix instructions: tail lanes and wall cells run with
+// zero features and store nothing.
+//
+// Training is hand-written too (float32): k_mlp_grad evaluates a minibatch,
+// forward and backward, one sample per lane, and reduces the per-sample
+// gradients of a workgroup through LDS in a fixed order; k_mlp_adam sums the
+// workgroup partials in a fixed order and applies Adam.  The result does not
+// depend on scheduling.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/cfd_mi355x.h"
+#include "kernels.h"
+#include "lattice.h"
+#include "lbm_device.h"
+
+namespace cfd {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ inference
+
+__device__ __forceinline__ float silu_fast(float a) {
+  return a * __builtin_amdgcn_rcpf(1.0f + __expf(-a));
+}
+
+__device__ __forceinline__ float softplus_fast(float z) {
+  if (z > 20.0f) return z;
+  if (z < -15.0f) return __expf(z);
+  return __logf(1.0f + __expf(z));
+}
+
+__device__ __forceinline__ v8bf ld8(const uint16_t* p) {
+  return *reinterpret_cast<const v8bf*>(p);
+}
+
+__device__ __forceinline__ v4f ld4(const float* p) { return *reinterpret_cast<const v4f*>(p); }
+
+__device__ __forceinline__ v8bf pack_silu(const v4f& a, const v4f& b) {
+  v8bf r;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    r[v] = (__bf16)silu_fast(a[v]);
+    r[v + 4] = (__bf16)silu_fast(b[v]);
+  }
+  return r;
+}
+
+// z (before softplus) of this lane's cell; xb = its normalized features in
+// bf16 (zero for lanes that do not use the result).  All 64 lanes must call.
+__device__ __forceinline__ float mlp_wave(const v8bf& xb, const MlpPacked& W) {
+  const int l = threadIdx.x & 63, grp = l >> 4, row = l & 15;
+  const v8bf zero = {};
+  const v4f zero4 = {0.f, 0.f, 0.f, 0.f};
+  float mine = 0.0f;
+#pragma unroll
+  for (int gs = 0; gs < 4; ++gs) {
+    const bool sel = grp == gs;
+    // layer 1, cells 16 gs .. 16 gs + 15: A = W1 rows in lane group gs only
+    const v8bf a0 = sel ? ld8(W.w1[0][row]) : zero;
+    const v8bf a1 = sel ? ld8(W.w1[1][row]) : zero;
+    const v4f h0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, xb, ld4(&W.b1[4 * grp]), 0, 0, 0);
+    const v4f h1 =
+        __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, xb, ld4(&W.b1[16 + 4 * grp]), 0, 0, 0);
+    const v8bf x2 = pack_silu(h0, h1);
+    // layer 2
+    const v4f g0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(&W.w2[0][row][8 * grp]), x2,
+                                                           ld4(&W.b2[4 * grp]), 0, 0, 0);
+    const v4f g1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(&W.w2[1][row][8 * grp]), x2,
+                                                           ld4(&W.b2[16 + 4 * grp]), 0, 0, 0);
+    const v8bf x3 = pack_silu(g0, g1);
+    // layer 3: row 0 of A is w3, the other rows zero
+    const v8bf a3 = row == 0 ? ld8(&W.w3[8 * grp]) : zero;
+    const v4f zz = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, x3, zero4, 0, 0, 0);
+    // the result of cell 16 gs + j sits in lane j
+    const float z = __shfl(zz[0], l - 16 * gs, 64);
+    if (sel) mine = z;
+  }
+  return mine + W.b3;
+}
+
+// Copy the packed MLP into LDS (all threads of the workgroup).
+__device__ __forceinline__ void load_mlp(MlpPacked& dst, const MlpPacked* __restrict__ src) {
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+  uint32_t* d = reinterpret_cast<uint32_t*>(&dst);
+  for (uint32_t k = threadIdx.x; k < sizeof(MlpPacked) / 4; k += blockDim.x) d[k] = s[k];
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ closure step
+
+template <class S, bool MACROS, int LB>
+__device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
+                                                  const GLOBAL typename S::T* __restrict__ src,
+                                                  GLOBAL typename S::T* __restrict__ dst,
+                                                  const GLOBAL uint8_t* __restrict__ tipo,
+                                                  const GLOBAL uint8_t* __restrict__ wall8,
+                                                  float4* __restrict__ macros, const Geom& g,
+                                                  const StepParams& p, const MlpPacked& W,
+                                                  unsigned long long* fallbacks) {
+  const uint32_t cell = live ? cell_in : g.N - 1;  // tail lanes read a valid cell
+  const Coord c = split(cell, g);
+  const Wrap w = wrap_offsets(c, g);
+
+  // pull with halfway bounce-back (ir.js:166-189), as in step_cell
+  const uint32_t tb = tipo[cell];
+  float gi[Q];
+  uint32_t sidx[Q];
+  gi[0] = S::ld(pop_at<LB>(src, 0, cell, g));
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    sidx[i] = (uint32_t)((int)cell + nb_offset(w, -cx(i), -cy(i), -cz(i)));
+    gi[i] = S::ld(pop_at<LB>(src, i, sidx[i], g));
+  }
+  const uint32_t ct = tb & TYPE_MASK;
+  const bool wall = is_wall(ct);
+  if (!wall && (tb & NEAR_WALL)) {
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const uint32_t t = tipo[sidx[i]] & TYPE_MASK;
+      if (is_wall(t)) {
+        const bool moving = (t == SOLIDO_MOVEL);
+        const float cu = dotc(i, moving ? p.belt[0] : 0.f, moving ? p.belt[1] : 0.f,
+                              moving ? p.belt[2] : 0.f);
+        gi[i] = S::ld(pop_at<LB>(src, opp(i), cell, g)) + kwall(i) * cu;
+      }
+    }
+  }
+
+  const Moments M = moments(gi);
+  float e[Q];
+  equilibrium(M, e);
+  float pab[6];
+  noneq_stress(gi, e, pab);
+  const float qs = stress_norm(pab);
+  const float tau_s = smagorinsky_tau(qs, M.rho, p);
+  const bool fluid = live && !wall && ct != ESPELHO_Y && ct != ESPELHO_Z && ct != ENTRADA &&
+                     ct != SAIDA;
+
+  // features, normalized; outside the trained box the cell keeps tau_s
+  float x[kMlpIn];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) x[k] = pab[k] / M.rho;
+  x[6] = 1.5f * qs / (M.rho * tau_s);
+  x[7] = (float)wall8[cell] * 0.125f;
+  bool inside = true;
+  v8bf xb;
+#pragma unroll
+  for (int k = 0; k < kMlpIn; ++k) {
+    const float xn = (x[k] - W.shift[k]) * W.scale[k];
+    inside = inside && xn >= W.lo[k] && xn <= W.hi[k];
+    xb[k] = (__bf16)xn;
+  }
+  const bool use = fluid && inside;
+  if (!use) xb = v8bf{};
+  const float nu = W.out_scale * softplus_fast(mlp_wave(xb, W));
+  const bool ok = use && isfinite(nu);
+  const unsigned long long fb = __ballot(fluid && !ok);
+  if (fb && (threadIdx.x & 63) == 0) atomicAdd(fallbacks, (unsigned long long)__popcll(fb));
+  const float tau = ok ? p.tau0 + 3.0f * nu : tau_s;
+
+  float d[Q];
+  if (!boundary_cell(ct, gi, M, c.z, p, d)) {
+    trt(gi, e, tau, p, d);
+    sponge(c.x, p, d);
+  }
+  if (live && !wall) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) S::st(pop_at<LB>(dst, i, cell, g), d[i]);
+  }
+  if (MACROS && live)
+    macros[cell] = wall ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(M.ux, M.uy, M.uz, M.delta);
+}
+
+constexpr int kClosureBlock = 256;
+
+template <class S, bool MACROS, int LB>
+__global__ __launch_bounds__(kClosureBlock) void k_step_closure(
+    const typename S::T* __restrict__ src, typename S::T* __restrict__ dst,
+    const uint8_t* __restrict__ tipo, const uint8_t* __restrict__ wall8,
+    const MlpPacked* __restrict__ mlp, float4* __restrict__ macros, Geom g, StepParams p,
+    unsigned long long* fallbacks) {
+  __shared__ MlpPacked W;
+  load_mlp(W, mlp);
+  const uint32_t cell = blockIdx.x * kClosureBlock + threadIdx.x;
+  step_cell_closure<S, MACROS, LB>(cell, cell < g.N, gptr(src), gptr(dst), gptr(tipo),
+                                   gptr(wall8), macros, g, p, W, fallbacks);
+}
+
+hipError_t launch_step_closure(Store sto, const void* src, void* dst, const uint8_t* tipo8,
+                               const uint8_t* wall8, const MlpPacked* mlp, float4* macros,
+                               bool write_macros, const Geom& g, const StepParams& p,
+                               unsigned long long* fallbacks, hipStream_t st) {
+  const dim3 grid((g.N + kClosureBlock - 1) / kClosureBlock);
+  CFD_DISPATCH(sto, g, {
+    if (write_macros)
+      hipLaunchKernelGGL((k_step_closure<S, true, L>), grid, dim3(kClosureBlock), 0, st,
+                         static_cast<const typename S::T*>(src), static_cast<typename S::T*>(dst),
+                         tipo8, wall8, mlp, macros, g, p, fallbacks);
+    else
+      hipLaunchKernelGGL((k_step_closure<S, false, L>), grid, dim3(kClosureBlock), 0, st,
+                         static_cast<const typename S::T*>(src), static_cast<typename S::T*>(dst),
+                         tipo8, wall8, mlp, macros, g, p, fallbacks);
+  });
+  return hipGetLastError();
+}
+
+// Inference alone on given raw features (the same device path as the step).
+__global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ X, uint32_t n,
+                                                  const MlpPacked* __restrict__ mlp,
+                                                  float* __restrict__ nu,
+                                                  uint8_t* __restrict__ inside_out) {
+  __shared__ MlpPacked W;
+  load_mlp(W, mlp);
+  const uint32_t s = blockIdx.x * 64 + threadIdx.x;
+  const bool live = s < n;
+  const uint32_t r = live ? s : 0;
+  bool inside = true;
+  v8bf xb;
+#pragma unroll
+  for (int k = 0; k < kMlpIn; ++k) {
+    const float xn = (X[(size_t)r * kMlpIn + k] - W.shift[k]) * W.scale[k];
+    inside = inside && xn >= W.lo[k] && xn <= W.hi[k];
+    xb[k] = (__bf16)xn;
+  }
+  const float out = W.out_scale * softplus_fast(mlp_wave(xb, W));
+  if (live) {
+    nu[s] = out;
+    inside_out[s] = inside ? 1 : 0;
+  }
+}
+
+// ------------------------------------------------------------------ training
+
+// Parameter offsets in the cfd_mlp order.
+constexpr int kOffW1 = 0, kOffB1 = 256, kOffW2 = 288, kOffB2 = 1312, kOffW3 = 1344,
+              kOffB3 = 1376;
+static_assert(kOffB3 + 1 == kMlpParams, "cfd_mlp layout");
+constexpr int kGradSlots = kMlpParams + 1;  // + the loss
+
+__device__ __forceinline__ float sigmoidf_exact(float a) { return 1.0f / (1.0f + expf(-a)); }
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// One workgroup = one wave = 64 samples.  random = 1: sample index drawn
+// from (seed, step, slot); random = 0: samples blockIdx.x * 64 + lane (full
+// batch).  inv_count = 1 / (samples in the mean).
+__global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ params,
+                                                 const float* __restrict__ X,
+                                                 const float* __restrict__ Y, uint64_t n,
+                                                 int random, uint64_t seed, uint32_t step,
+                                                 float inv_count, float* __restrict__ partial) {
+  __shared__ float P[kMlpParams];
+  __shared__ float xs[64][kMlpIn];
+  __shared__ float h1s[64][kMlpHidden], d1s[64][kMlpHidden];
+  __shared__ float h2s[64][kMlpHidden], d2s[64][kMlpHidden];
+  __shared__ float dzs[64], ls[64];
+  const int t = threadIdx.x;
+  for (int k = t; k < kMlpParams; k += 64) P[k] = params[k];
+  __syncthreads();
+
+  uint64_t s;
+  bool live;
+  if (random) {
+    s = mix64(seed ^ ((uint64_t)step << 32) ^ (uint64_t)(blockIdx.x * 64 + t)) % n;
+    live = true;
+  } else {
+    s = (uint64_t)blockIdx.x * 64 + t;
+    live = s < n;
+    if (!live) s = 0;
+  }
+  float x[kMlpIn];
+#pragma unroll
+  for (int k = 0; k < kMlpIn; ++k) x[k] = X[s * kMlpIn + k];
+  const float y = Y[s];
+
+  // forward
+  float a1[kMlpHidden], h1[kMlpHidden], a2[kMlpHidden], h2[kMlpHidden];
+#pragma unroll
+  for (int o = 0; o < kMlpHidden; ++o) {
+    float acc = P[kOffB1 + o];
+#pragma unroll
+    for (int k = 0; k < kMlpIn; ++k) acc += P[kOffW1 + o * kMlpIn + k] * x[k];
+    a1[o] = acc;
+    h1[o] = acc * sigmoidf_exact(acc);
+  }
+#pragma unroll
+  for (int o = 0; o < kMlpHidden; ++o) {
+    float acc = P[kOffB2 + o];
+#pragma unroll
+    for (int k = 0; k < kMlpHidden; ++k) acc += P[kOffW2 + o * kMlpHidden + k] * h1[k];
+    a2[o] = acc;
+    h2[o] = acc * sigmoidf_exact(acc);
+  }
+  float z = P[kOffB3];
+#pragma unroll
+  for (int k = 0; k < kMlpHidden; ++k) z += P[kOffW3 + k] * h2[k];
+  const float yhat = z > 20.0f ? z : log1pf(expf(z));
+  const float err = yhat - y;
+
+  // backward: L = mean (yhat - y)^2
+  const float dz = live ? 2.0f * err * inv_count * sigmoidf_exact(z) : 0.0f;
+#pragma unroll
+  for (int o = 0; o < kMlpHidden; ++o) {
+    const float sg = sigmoidf_exact(a2[o]);
+    d2s[t][o] = dz * P[kOffW3 + o] * (sg * (1.0f + a2[o] * (1.0f - sg)));
+    h2s[t][o] = h2[o];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kMlpHidden; ++k) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int o = 0; o < kMlpHidden; ++o) acc += P[kOffW2 + o * kMlpHidden + k] * d2s[t][o];
+    const float sg = sigmoidf_exact(a1[k]);
+    d1s[t][k] = acc * (sg * (1.0f + a1[k] * (1.0f - sg)));
+    h1s[t][k] = h1[k];
+  }
+#pragma unroll
+  for (int k = 0; k < kMlpIn; ++k) xs[t][k] = x[k];
+  dzs[t] = dz;
+  ls[t] = live ? err * err : 0.0f;
+  __syncthreads();
+
+  // workgroup sums over the 64 samples, in sample order
+  float* out = partial + (size_t)blockIdx.x * kGradSlots;
+  for (int q = t; q < kGradSlots; q += 64) {
+    float acc = 0.0f;
+    if (q < kOffB1) {
+      const int o = q / kMlpIn, k = q % kMlpIn;
+      for (int r = 0; r < 64; ++r) acc += d1s[r][o] * xs[r][k];
+    } else if (q < kOffW2) {
+      const int o = q - kOffB1;
+      for (int r = 0; r < 64; ++r) acc += d1s[r][o];
+    } else if (q < kOffB2) {
+      const int o = (q - kOffW2) / kMlpHidden, k = (q - kOffW2) % kMlpHidden;
+      for (int r = 0; r < 64; ++r) acc += d2s[r][o] * h1s[r][k];
+    } else if (q < kOffW3) {
+      const int o = q - kOffB2;
+      for (int r = 0; r < 64; ++r) acc += d2s[r][o];
+    } else if (q < kOffB3) {
+      const int k = q - kOffW3;
+      for (int r = 0; r < 64; ++r) acc += dzs[r] * h2s[r][k];
+    } else if (q == kOffB3) {
+      for (int r = 0; r < 64; ++r) acc += dzs[r];
+    } else {
+      for (int r = 0; r < 64; ++r) acc += ls[r];
+    }
+    out[q] = acc;
+  }
+}
+
+// Sum of the workgroup partials (fixed order) -> gradient, then Adam when
+// lr > 0.  Slot kMlpParams carries the summed squared error.
+__global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ params,
+                                                  float* __restrict__ m, float* __restrict__ v,
+                                                  const float* __restrict__ partial, int nwg,
+                                                  uint32_t step, float lr, float beta1,
+                                                  float beta2, float eps, float loss_scale,
+                                                  float* __restrict__ grad_out,
+                                                  float* __restrict__ loss_out) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= kGradSlots) return;
+  float gsum = 0.0f;
+  for (int w = 0; w < nwg; ++w) gsum += partial[(size_t)w * kGradSlots + q];
+  if (q == kMlpParams) {
+    if (loss_out) loss_out[step - 1] = gsum * loss_scale;
+    return;
+  }
+  if (grad_out) grad_out[q] = gsum;
+  if (lr > 0.0f) {
+    const float mq = beta1 * m[q] + (1.0f - beta1) * gsum;
+    const float vq = beta2 * v[q] + (1.0f - beta2) * gsum * gsum;
+    m[q] = mq;
+    v[q] = vq;
+    const float mh = mq / (1.0f - powf(beta1, (float)step));
+    const float vh = vq / (1.0f - powf(beta2, (float)step));
+    params[q] -= lr * mh / (sqrtf(vh) + eps);
+  }
+}
+
+}  // namespace cfd
+
+// ------------------------------------------------------------------ host side
+
+using namespace cfd;
+
+extern "C" void cfd__set_error(const char* msg);
+
+namespace {
+
+uint16_t bf16_rne(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+int perm_k(int k) {
+  const int q = k >> 3, v = k & 7;
+  return v < 4 ? 4 * q + v : 16 + 4 * q + (v - 4);
+}
+
+int efail(int code, const char* what, hipError_t e) {
+  char buf[256];
+  snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  cfd__set_error(buf);
+  return code;
+}
+
+}  // namespace
+
+namespace cfd {
+
+void pack_closure(const cfd_closure& c, MlpPacked* out) {
+  std::memset(out, 0, sizeof *out);
+  const cfd_mlp& m = c.mlp;
+  for (int t = 0; t < 2; ++t)
+    for (int i = 0; i < 16; ++i) {
+      for (int v = 0; v < 8; ++v) out->w1[t][i][v] = bf16_rne(m.w1[(16 * t + i) * 8 + v]);
+      for (int k = 0; k < 32; ++k) out->w2[t][i][k] = bf16_rne(m.w2[(16 * t + i) * 32 + perm_k(k)]);
+    }
+  for (int k = 0; k < 32; ++k) out->w3[k] = bf16_rne(m.w3[perm_k(k)]);
+  for (int o = 0; o < 32; ++o) {
+    out->b1[o] = m.b1[o];
+    out->b2[o] = m.b2[o];
+  }
+  out->b3 = m.b3;
+  out->out_scale = c.out_scale;
+  for (int k = 0; k < 8; ++k) {
+    out->shift[k] = c.in_shift[k];
+    out->scale[k] = c.in_scale[k];
+    out->lo[k] = c.in_lo[k];
+    out->hi[k] = c.in_hi[k];
+  }
+}
+
+}  // namespace cfd
+
+extern "C" {
+
+int cfd_mlp_infer(const float* X, size_t n, const cfd_closure* c, float* nu, uint8_t* inside) {
+  cfd__set_error("");
+  if (!X || !c || !nu || !inside || n == 0 || n >= (1ull << 31)) {
+    cfd__set_error("cfd_mlp_infer: bad arguments");
+    return CFD_E_ARG;
+  }
+  MlpPacked pk;
+  pack_closure(*c, &pk);
+  float *dX = nullptr, *dnu = nullptr;
+  uint8_t* din = nullptr;
+  MlpPacked* dm = nullptr;
+  hipError_t e = hipMalloc(&dX, n * kMlpIn * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dnu, n * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&din, n);
+  if (e == hipSuccess) e = hipMalloc(&dm, sizeof pk);
+  if (e == hipSuccess) e = hipMemcpy(dX, X, n * kMlpIn * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dm, &pk, sizeof pk, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, 0, dX,
+                       (uint32_t)n, dm, dnu, din);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(nu, dnu, n * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(inside, din, n, hipMemcpyDeviceToHost);
+  (void)hipFree(dX);
+  (void)hipFree(dnu);
+  (void)hipFree(din);
+  (void)hipFree(dm);
+  if (e != hipSuccess) return efail(e == hipErrorOutOfMemory ? CFD_E_OOM : CFD_E_HIP,
+                                    "cfd_mlp_infer", e);
+  return CFD_OK;
+}
+
+// Shared driver of cfd_mlp_train / cfd_mlp_grad.
+static int mlp_run(const float* X, const float* Y, size_t n, const cfd_train_cfg* cfg,
+                   cfd_mlp* params, cfd_mlp* grad, double* loss_full, float* loss_curve) {
+  const bool train = cfg != nullptr;
+  const int batch = train ? cfg->batch : 0;
+  const int nwg = train ? batch / 64 : (int)((n + 63) / 64);
+  float *dX = nullptr, *dY = nullptr, *dP = nullptr, *dm = nullptr, *dv = nullptr,
+        *dpart = nullptr, *dgrad = nullptr, *dloss = nullptr;
+  const int steps = train ? cfg->steps : 1;
+  hipError_t e = hipMalloc(&dX, n * kMlpIn * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dY, n * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dP, kMlpParams * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dm, kMlpParams * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dv, kMlpParams * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dpart, (size_t)nwg * kGradSlots * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dgrad, kMlpParams * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dloss, (size_t)steps * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(dX, X, n * kMlpIn * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dY, Y, n * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dP, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(dm, 0, kMlpParams * sizeof(float));
+  if (e == hipSuccess) e = hipMemset(dv, 0, kMlpParams * sizeof(float));
+  const unsigned agrid = (kGradSlots + 255) / 256;
+  for (int s = 1; e == hipSuccess && s <= steps; ++s) {
+    if (train) {
+      hipLaunchKernelGGL(k_mlp_grad, dim3(nwg), dim3(64), 0, 0, dP, dX, dY, (uint64_t)n, 1,
+                         (uint64_t)cfg->seed, (uint32_t)s, 1.0f / (float)batch, dpart);
+      hipLaunchKernelGGL(k_mlp_adam, dim3(agrid), dim3(256), 0, 0, dP, dm, dv, dpart, nwg,
+                         (uint32_t)s, cfg->lr, cfg->beta1, cfg->beta2, cfg->eps,
+                         1.0f / (float)batch, (float*)nullptr, dloss);
+    } else {
+      hipLaunchKernelGGL(k_mlp_grad, dim3(nwg), dim3(64), 0, 0, dP, dX, dY, (uint64_t)n, 0,
+                         (uint64_t)0, 1u, 1.0f / (float)n, dpart);
+      hipLaunchKernelGGL(k_mlp_adam, dim3(agrid), dim3(256), 0, 0, dP, dm, dv, dpart, nwg, 1u,
+                         0.0f, 0.9f, 0.999f, 1e-8f, 1.0f / (float)n, dgrad, dloss);
+    }
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess && train)
+    e = hipMemcpy(params, dP, kMlpParams * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && train && loss_curve)
+    e = hipMemcpy(loss_curve, dloss, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && !train && grad)
+    e = hipMemcpy(grad, dgrad, kMlpParams * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && !train && loss_full) {
+    float l = 0.f;
+    e = hipMemcpy(&l, dloss, sizeof(float), hipMemcpyDeviceToHost);
+    *loss_full = l;
+  }
+  for (float* p : {dX, dY, dP, dm, dv, dpart, dgrad, dloss}) (void)hipFree(p);
+  if (e != hipSuccess) return efail(e == hipErrorOutOfMemory ? CFD_E_OOM : CFD_E_HIP,
+                                    train ? "cfd_mlp_train" : "cfd_mlp_grad", e);
+  return CFD_OK;
+}
+
+int cfd_mlp_train(const float* X, const float* y, size_t n, const cfd_train_cfg* cfg,
+                  cfd_mlp* params, float* loss_curve) {
+  cfd__set_error("");
+  if (!X || !y || !cfg || !params || n == 0 || cfg->steps < 1 || cfg->batch < 64 ||
+      cfg->batch % 64 != 0 || !(cfg->lr > 0.0f)) {
+    cfd__set_error("cfd_mlp_train: bad arguments (batch a multiple of 64, steps >= 1, lr > 0)");
+    return CFD_E_ARG;
+  }
+  return mlp_run(X, y, n, cfg, params, nullptr, nullptr, loss_curve);
+}
+
+int cfd_mlp_grad(const float* X, const float* y, size_t n, const cfd_mlp* params, cfd_mlp* grad,
+                 double* loss) {
+  cfd__set_error("");
+  if (!X || !y || !params || !grad || n == 0 || n >= (1ull << 31)) {
+    cfd__set_error("cfd_mlp_grad: bad arguments");
+    return CFD_E_ARG;
+  }
+  cfd_mlp p = *params;
+  return mlp_run(X, y, n, nullptr, &p, grad, loss, nullptr);
+}
+
+}  // extern "C"

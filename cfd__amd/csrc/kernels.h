@@ -120,4 +120,28 @@ hipError_t batch_max_grid(int block, int* max_blocks_per_cu, int* cus);
 hipError_t launch_batch_run(const BatchTunnel* tun, const BatchLaunch& L, int block, int grid,
                             hipStream_t st);
 
+// ---- learned closure (closure.hip) ----
+constexpr int kMlpIn = 8, kMlpHidden = 32;
+constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + kMlpHidden * kMlpHidden +
+                           kMlpHidden + kMlpHidden + 1;  // 1377, the cfd_mlp order
+// The MLP in the operand layouts of v_mfma_f32_16x16x32_bf16, built on the
+// host (bf16 = round-to-nearest-even of the float32 weights).  Hidden units
+// of a layer's output reach the next layer in the permuted order
+// k = 8q + v -> (v < 4 ? 4q + v : 16 + 4q + v - 4), which is how a 16x16
+// result tile is spread over the lanes; w2 and w3 are stored with their input
+// index permuted accordingly.
+struct MlpPacked {
+  uint16_t w1[2][16][8];    // M-tile m, row i = hidden 16m + i, input v
+  uint16_t w2[2][16][32];   // M-tile m, row i = output 16m + i, permuted input k
+  uint16_t w3[32];          // permuted input k
+  float b1[32], b2[32];
+  float b3, out_scale;
+  float shift[8], scale[8], lo[8], hi[8];
+};
+void pack_closure(const ::cfd_closure& c, MlpPacked* out);
+hipError_t launch_step_closure(Store sto, const void* src, void* dst, const uint8_t* tipo8,
+                               const uint8_t* wall8, const MlpPacked* mlp, float4* macros,
+                               bool write_macros, const Geom& g, const StepParams& p,
+                               unsigned long long* fallbacks, hipStream_t st);
+
 }  // namespace cfd

@@ -1,0 +1,347 @@
+// Device helpers shared by the lattice kernels (lbm_kernels.hip) and the
+// learned-closure kernels (closure.hip): storage formats, addressing, the
+// (x, y, z) split with periodic neighbour offsets, and the per-cell pieces of
+// the emitted step (moments, equilibrium, non-equilibrium stress, TRT pair
+// collision), written once so that every kernel rounds identically.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+#include "lattice.h"
+
+namespace cfd {
+
+// ------------------------------------------------------------------ storage
+
+// Global address space.  Pointers that the kernels read from device memory
+// (the batch descriptors) carry no address space; accessing through a
+// GLOBAL-qualified pointer keeps their loads and stores in the global form
+// with a wave-uniform base in SGPRs instead of FLAT with per-lane 64-bit
+// addresses.
+#define GLOBAL __attribute__((address_space(1)))
+
+template <class T>
+__device__ __forceinline__ const GLOBAL T* gptr(const T* p) {
+  return (const GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ GLOBAL T* gptr(T* p) {
+  return (GLOBAL T*)p;
+}
+
+struct StoreF32 {
+  using T = float;
+  template <class P>
+  static __device__ __forceinline__ float ld(P p) { return *p; }
+  template <class P>
+  static __device__ __forceinline__ void st(P p, float v) { *p = v; }
+};
+
+// float16 deviations g_i = f_i - w_i scaled by 2^12 (exact power of two, so
+// the scaling itself adds no rounding): |g| up to 16 stays finite and values
+// down to 1.5e-8 stay normal.  Collision still runs in float32.
+struct StoreF16 {
+  using T = _Float16;
+  template <class P>
+  static __device__ __forceinline__ float ld(P p) {
+    return static_cast<float>(*p) * 0.000244140625f;
+  }
+  template <class P>
+  static __device__ __forceinline__ void st(P p, float v) { *p = static_cast<T>(v * 4096.0f); }
+};
+
+// ------------------------------------------------------------------ indexing
+
+// Element `idx` of the slice starting at `base`: a wave-uniform 64-bit base
+// plus a 32-bit byte offset, so the load uses the scalar-base addressing form
+// (no per-lane 64-bit address arithmetic).  idx * sizeof(T) < 2^32 because a
+// handle holds fewer than 2^30 cells.
+template <class T>
+__device__ __forceinline__ const T* at(const T* base, uint32_t idx) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                    (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ T* at(T* base, uint32_t idx) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) +
+                              (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ const GLOBAL T* at(const GLOBAL T* base, uint32_t idx) {
+  return (const GLOBAL T*)((const GLOBAL char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ GLOBAL T* at(GLOBAL T* base, uint32_t idx) {
+  return (GLOBAL T*)((GLOBAL char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+
+// Address of population i of cell c.  LB = 0: SoA, slice stride g.S, a
+// wave-uniform 64-bit slice base plus a 32-bit offset.  LB > 0: blocks of
+// 2^LB cells, pop[(c >> LB)][i][c & (2^LB - 1)], a 32-bit byte offset from
+// the set's base (the host only selects it when a population set is below
+// 4 GiB).
+template <int LB, class T>
+__device__ __forceinline__ T* pop_at(T* base, int i, uint32_t c, const Geom& g) {
+  if constexpr (LB == 0) {
+    return at(base + (size_t)i * g.S, c);
+  } else {
+    constexpr uint32_t B = 1u << LB;
+    return at(base, (c >> LB) * (B * (uint32_t)Q) + (uint32_t)i * B + (c & (B - 1)));
+  }
+}
+
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, uint32_t s) {
+  return (__umulhi(n, m) + n) >> s;
+}
+
+// Logical workgroup of this launch: physical workgroup b runs on XCD b % 8;
+// XCD k owns logical workgroups [k*per_xcd, (k+1)*per_xcd).
+__device__ __forceinline__ uint32_t logical_block(const Geom& g) {
+  const uint32_t b = blockIdx.x;
+  return (b % kXcds) * g.per_xcd + b / kXcds;
+}
+__device__ __forceinline__ uint32_t logical_block(uint32_t per_xcd, int remap) {
+  const uint32_t b = blockIdx.x;
+  return remap ? (b % kXcds) * per_xcd + b / kXcds : b;
+}
+
+struct Coord {
+  uint32_t x, y, z;
+};
+
+__device__ __forceinline__ Coord split(uint32_t cell, const Geom& g) {
+  Coord c;
+  c.z = fast_div(cell, g.mnxy, g.snxy);
+  const uint32_t r = cell - c.z * g.nxny;
+  c.y = fast_div(r, g.mnx, g.snx);
+  c.x = r - c.y * g.nx;
+  return c;
+}
+
+// Signed index offsets to the periodic neighbours x-1, x+1 (and y, z), the
+// select-based wrap of wgsl.js:159-182 (no modulo).
+struct Wrap {
+  int xm, xp, ym, yp, zm, zp;
+};
+
+__device__ __forceinline__ Wrap wrap_offsets(const Coord& c, const Geom& g) {
+  Wrap w;
+  const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz, nxny = (int)g.nxny;
+  w.xm = c.x == 0 ? nx - 1 : -1;
+  w.xp = c.x == g.nx - 1 ? -(nx - 1) : 1;
+  w.ym = c.y == 0 ? (ny - 1) * nx : -nx;
+  w.yp = c.y == g.ny - 1 ? -(ny - 1) * nx : nx;
+  w.zm = c.z == 0 ? (nz - 1) * nxny : -nxny;
+  w.zp = c.z == g.nz - 1 ? -(nz - 1) * nxny : nxny;
+  return w;
+}
+
+// Offset of the cell at x + (dx, dy, dz), each component in {-1, 0, 1}.
+__device__ __forceinline__ int nb_offset(const Wrap& w, int dx, int dy, int dz) {
+  const int ox = dx < 0 ? w.xm : (dx > 0 ? w.xp : 0);
+  const int oy = dy < 0 ? w.ym : (dy > 0 ? w.yp : 0);
+  const int oz = dz < 0 ? w.zm : (dz > 0 ? w.zp : 0);
+  return ox + oy + oz;
+}
+
+// c_i . v written as the emitted literal arithmetic (ir.js:116-126): the
+// non-zero components in x, y, z order, a leading minus for a negative first
+// component, then + or -.
+__device__ __forceinline__ float dotc(int i, float vx, float vy, float vz) {
+  float acc = 0.0f;
+  bool first = true;
+  const int c[3] = {cx(i), cy(i), cz(i)};
+  const float v[3] = {vx, vy, vz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (c[a] == 0) continue;
+    if (first) {
+      acc = c[a] > 0 ? v[a] : -v[a];
+      first = false;
+    } else {
+      acc = c[a] > 0 ? acc + v[a] : acc - v[a];
+    }
+  }
+  return acc;
+}
+
+// ------------------------------------------------------------------ cell physics
+
+// Moments of the pulled populations (ir.js:192-213), left-to-right sums as
+// emitted: delta = sum g_i, rho = 1 + delta, m = sum c_i g_i.
+struct Moments {
+  float delta, rho, ux, uy, uz, uu;
+};
+
+__device__ __forceinline__ Moments moments(const float (&gi)[Q]) {
+  Moments M;
+  float delta = gi[0];
+#pragma unroll
+  for (int i = 1; i < Q; ++i) delta = delta + gi[i];
+  M.delta = delta;
+  M.rho = 1.0f + delta;
+  float m[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float acc = 0.0f;
+    bool first = true;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const int ca = a == 0 ? cx(i) : (a == 1 ? cy(i) : cz(i));
+      if (ca == 0) continue;
+      if (first) {
+        acc = ca > 0 ? gi[i] : -gi[i];
+        first = false;
+      } else {
+        acc = ca > 0 ? acc + gi[i] : acc - gi[i];
+      }
+    }
+    m[a] = acc;
+  }
+  M.ux = m[0] / M.rho;
+  M.uy = m[1] / M.rho;
+  M.uz = m[2] / M.rho;
+  M.uu = M.ux * M.ux + M.uy * M.uy + M.uz * M.uz;
+  return M;
+}
+
+// Shifted equilibrium (ir.js:216-232).
+__device__ __forceinline__ void equilibrium(const Moments& M, float (&e)[Q]) {
+  e[0] = wf(0) * (M.delta - 1.5f * M.rho * M.uu);
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    const float cu = dotc(i, M.ux, M.uy, M.uz);
+    e[i] = wf(i) * (M.delta + M.rho * (3.0f * cu + 4.5f * cu * cu - 1.5f * M.uu));
+  }
+}
+
+// Non-equilibrium momentum flux Pi_ab = sum c_ia c_ib (g_i - e_i) in the
+// order xx, yy, zz, xy, xz, yz (ir.js:262-285).
+__device__ __forceinline__ void noneq_stress(const float (&gi)[Q], const float (&e)[Q],
+                                             float (&pab)[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int a = k < 3 ? k : (k == 5 ? 1 : 0);
+    const int b = k < 3 ? k : (k == 3 ? 1 : 2);
+    float acc = 0.0f;
+    bool first = true;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int ca = a == 0 ? cx(i) : (a == 1 ? cy(i) : cz(i));
+      const int cb = b == 0 ? cx(i) : (b == 1 ? cy(i) : cz(i));
+      const int coef = ca * cb;
+      if (coef == 0) continue;
+      const float n = gi[i] - e[i];
+      if (first) {
+        acc = coef > 0 ? n : -n;
+        first = false;
+      } else {
+        acc = coef > 0 ? acc + n : acc - n;
+      }
+    }
+    pab[k] = acc;
+  }
+}
+
+// Q_s = sqrt(2 Pi:Pi) (ir.js:286-289).
+__device__ __forceinline__ float stress_norm(const float (&pab)[6]) {
+  const float pp = pab[0] * pab[0] + pab[1] * pab[1] + pab[2] * pab[2] +
+                   2.0f * (pab[3] * pab[3] + pab[4] * pab[4] + pab[5] * pab[5]);
+  return sqrtf(2.0f * pp);
+}
+
+// Closed-form Smagorinsky total relaxation time (ir.js:290-297).
+__device__ __forceinline__ float smagorinsky_tau(float qs, float rho, const StepParams& p) {
+  return 0.5f * (p.tau0 + sqrtf(p.tau0sq + p.c18cs2 * qs / rho));
+}
+
+// TRT collision per opposite pair with omega- from the fixed magic
+// parameter (ir.js:298-323).
+__device__ __forceinline__ void trt(const float (&gi)[Q], const float (&e)[Q], float tau,
+                                    const StepParams& p, float (&d)[Q]) {
+  const float omP = 1.0f / tau;
+  const float omM = 1.0f / (p.magic / (1.0f / omP - 0.5f) + 0.5f);
+  d[0] = gi[0] - omP * (gi[0] - e[0]);
+#pragma unroll
+  for (int i = 1; i < Q; i += 2) {
+    const int j = i + 1;
+    const float pe = 0.5f * ((gi[i] + gi[j]) - (e[i] + e[j]));
+    const float mo = 0.5f * ((gi[i] - gi[j]) - (e[i] - e[j]));
+    d[i] = gi[i] - omP * pe - omM * mo;
+    d[j] = gi[j] - omP * pe + omM * mo;
+  }
+}
+
+// Sponge (ir.js:458-478): mix(dst, teq, sigma^2).  For x <= start sigma^2 = 0
+// and mix returns dst unchanged, so the blend is skipped.
+__device__ __forceinline__ void sponge(uint32_t x, const StepParams& p, float (&d)[Q]) {
+  const float dxs = (float)x - p.sponge_start;
+  if (dxs > 0.0f) {
+    const float sigma = fminf(dxs / p.sponge_len_m, 1.0f);
+    const float sig2 = sigma * sigma;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) d[i] = d[i] * (1.0f - sig2) + p.tinf[i] * sig2;
+  }
+}
+
+// The non-fluid branches of emitirPasso (ir.js:506-529): free-slip mirrors,
+// the inlet with its 1/7-power profile (wgsl.js:213-220) and the outlet.
+// Returns false for the fluid fall-through.
+__device__ __forceinline__ bool boundary_cell(uint32_t ct, const float (&gi)[Q], const Moments& M,
+                                              uint32_t z, const StepParams& p, float (&d)[Q]) {
+  if (ct == ESPELHO_Y) {
+    // free-slip mirror (ir.js:383-390): reflect the y component
+#pragma unroll
+    for (int i = 0; i < Q; ++i) d[i] = gi[mirror(i, 1)];
+    return true;
+  }
+  if (ct == ESPELHO_Z) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) d[i] = gi[mirror(i, 2)];
+    return true;
+  }
+  if (ct == ENTRADA) {
+    // inlet (ir.js:405-422): equilibrium at the prescribed velocity,
+    // delta = 0, rho = 1
+    const float zc = (float)z + 0.5f;
+    const float blf = zc < p.bl ? powf(zc / p.bl_m, 0.1428571429f) : 1.0f;
+    const float ex = p.inlet[0] * blf, ey = p.inlet[1] * blf, ez = p.inlet[2] * blf;
+    const float uuin = ex * ex + ey * ey + ez * ez;
+    d[0] = wf(0) * (0.0f - 1.5f * 1.0f * uuin);
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const float ci = dotc(i, ex, ey, ez);
+      d[i] = wf(i) * (0.0f + 1.0f * (3.0f * ci + 4.5f * ci * ci - 1.5f * uuin));
+    }
+    return true;
+  }
+  if (ct == SAIDA) {
+    // outlet (ir.js:432-444): equilibrium at the local u with delta = 0
+    d[0] = wf(0) * (-1.5f * M.uu);
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const float cu = dotc(i, M.ux, M.uy, M.uz);
+      d[i] = wf(i) * (3.0f * cu + 4.5f * cu * cu - 1.5f * M.uu);
+    }
+    return true;
+  }
+  return false;
+}
+
+// Instantiate a block for each storage format (S) and layout (L).
+#define CFD_DISPATCH(sto, g, ...)                                          \
+  do {                                                                     \
+    if ((sto) == Store::F32) {                                             \
+      using S = StoreF32;                                                  \
+      if ((g).lb) { constexpr int L = kBlockLog2; __VA_ARGS__ }            \
+      else { constexpr int L = 0; __VA_ARGS__ }                            \
+    } else {                                                               \
+      using S = StoreF16;                                                  \
+      if ((g).lb) { constexpr int L = kBlockLog2; __VA_ARGS__ }            \
+      else { constexpr int L = 0; __VA_ARGS__ }                            \
+    }                                                                      \
+  } while (0)
+
+}  // namespace cfd

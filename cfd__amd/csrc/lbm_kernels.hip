@@ -25,162 +25,9 @@
 
 #include "kernels.h"
 #include "lattice.h"
+#include "lbm_device.h"
 
 namespace cfd {
-
-// ------------------------------------------------------------------ storage
-
-// Global address space.  Pointers that the kernels read from device memory
-// (the batch descriptors) carry no address space; accessing through a
-// GLOBAL-qualified pointer keeps their loads and stores in the global form
-// with a wave-uniform base in SGPRs instead of FLAT with per-lane 64-bit
-// addresses.
-#define GLOBAL __attribute__((address_space(1)))
-
-template <class T>
-__device__ __forceinline__ const GLOBAL T* gptr(const T* p) {
-  return (const GLOBAL T*)p;
-}
-template <class T>
-__device__ __forceinline__ GLOBAL T* gptr(T* p) {
-  return (GLOBAL T*)p;
-}
-
-struct StoreF32 {
-  using T = float;
-  template <class P>
-  static __device__ __forceinline__ float ld(P p) { return *p; }
-  template <class P>
-  static __device__ __forceinline__ void st(P p, float v) { *p = v; }
-};
-
-// float16 deviations g_i = f_i - w_i scaled by 2^12 (exact power of two, so
-// the scaling itself adds no rounding): |g| up to 16 stays finite and values
-// down to 1.5e-8 stay normal.  Collision still runs in float32.
-struct StoreF16 {
-  using T = _Float16;
-  template <class P>
-  static __device__ __forceinline__ float ld(P p) {
-    return static_cast<float>(*p) * 0.000244140625f;
-  }
-  template <class P>
-  static __device__ __forceinline__ void st(P p, float v) { *p = static_cast<T>(v * 4096.0f); }
-};
-
-// ------------------------------------------------------------------ indexing
-
-// Element `idx` of the slice starting at `base`: a wave-uniform 64-bit base
-// plus a 32-bit byte offset, so the load uses the scalar-base addressing form
-// (no per-lane 64-bit address arithmetic).  idx * sizeof(T) < 2^32 because a
-// handle holds fewer than 2^30 cells.
-template <class T>
-__device__ __forceinline__ const T* at(const T* base, uint32_t idx) {
-  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
-                                    (uint32_t)(idx * (uint32_t)sizeof(T)));
-}
-template <class T>
-__device__ __forceinline__ T* at(T* base, uint32_t idx) {
-  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) +
-                              (uint32_t)(idx * (uint32_t)sizeof(T)));
-}
-template <class T>
-__device__ __forceinline__ const GLOBAL T* at(const GLOBAL T* base, uint32_t idx) {
-  return (const GLOBAL T*)((const GLOBAL char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
-}
-template <class T>
-__device__ __forceinline__ GLOBAL T* at(GLOBAL T* base, uint32_t idx) {
-  return (GLOBAL T*)((GLOBAL char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
-}
-
-// Address of population i of cell c.  LB = 0: SoA, slice stride g.S, a
-// wave-uniform 64-bit slice base plus a 32-bit offset.  LB > 0: blocks of
-// 2^LB cells, pop[(c >> LB)][i][c & (2^LB - 1)], a 32-bit byte offset from
-// the set's base (the host only selects it when a population set is below
-// 4 GiB).
-template <int LB, class T>
-__device__ __forceinline__ T* pop_at(T* base, int i, uint32_t c, const Geom& g) {
-  if constexpr (LB == 0) {
-    return at(base + (size_t)i * g.S, c);
-  } else {
-    constexpr uint32_t B = 1u << LB;
-    return at(base, (c >> LB) * (B * (uint32_t)Q) + (uint32_t)i * B + (c & (B - 1)));
-  }
-}
-
-__device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, uint32_t s) {
-  return (__umulhi(n, m) + n) >> s;
-}
-
-// Logical workgroup of this launch: physical workgroup b runs on XCD b % 8;
-// XCD k owns logical workgroups [k*per_xcd, (k+1)*per_xcd).
-__device__ __forceinline__ uint32_t logical_block(const Geom& g) {
-  const uint32_t b = blockIdx.x;
-  return (b % kXcds) * g.per_xcd + b / kXcds;
-}
-__device__ __forceinline__ uint32_t logical_block(uint32_t per_xcd, int remap) {
-  const uint32_t b = blockIdx.x;
-  return remap ? (b % kXcds) * per_xcd + b / kXcds : b;
-}
-
-struct Coord {
-  uint32_t x, y, z;
-};
-
-__device__ __forceinline__ Coord split(uint32_t cell, const Geom& g) {
-  Coord c;
-  c.z = fast_div(cell, g.mnxy, g.snxy);
-  const uint32_t r = cell - c.z * g.nxny;
-  c.y = fast_div(r, g.mnx, g.snx);
-  c.x = r - c.y * g.nx;
-  return c;
-}
-
-// Signed index offsets to the periodic neighbours x-1, x+1 (and y, z), the
-// select-based wrap of wgsl.js:159-182 (no modulo).
-struct Wrap {
-  int xm, xp, ym, yp, zm, zp;
-};
-
-__device__ __forceinline__ Wrap wrap_offsets(const Coord& c, const Geom& g) {
-  Wrap w;
-  const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz, nxny = (int)g.nxny;
-  w.xm = c.x == 0 ? nx - 1 : -1;
-  w.xp = c.x == g.nx - 1 ? -(nx - 1) : 1;
-  w.ym = c.y == 0 ? (ny - 1) * nx : -nx;
-  w.yp = c.y == g.ny - 1 ? -(ny - 1) * nx : nx;
-  w.zm = c.z == 0 ? (nz - 1) * nxny : -nxny;
-  w.zp = c.z == g.nz - 1 ? -(nz - 1) * nxny : nxny;
-  return w;
-}
-
-// Offset of the cell at x + (dx, dy, dz), each component in {-1, 0, 1}.
-__device__ __forceinline__ int nb_offset(const Wrap& w, int dx, int dy, int dz) {
-  const int ox = dx < 0 ? w.xm : (dx > 0 ? w.xp : 0);
-  const int oy = dy < 0 ? w.ym : (dy > 0 ? w.yp : 0);
-  const int oz = dz < 0 ? w.zm : (dz > 0 ? w.zp : 0);
-  return ox + oy + oz;
-}
-
-// c_i . v written as the emitted literal arithmetic (ir.js:116-126): the
-// non-zero components in x, y, z order, a leading minus for a negative first
-// component, then + or -.
-__device__ __forceinline__ float dotc(int i, float vx, float vy, float vz) {
-  float acc = 0.0f;
-  bool first = true;
-  const int c[3] = {cx(i), cy(i), cz(i)};
-  const float v[3] = {vx, vy, vz};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (c[a] == 0) continue;
-    if (first) {
-      acc = c[a] > 0 ? v[a] : -v[a];
-      first = false;
-    } else {
-      acc = c[a] > 0 ? acc + v[a] : acc - v[a];
-    }
-  }
-  return acc;
-}
 
 // ------------------------------------------------------------------ types
 
@@ -275,124 +122,22 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
     }
   }
 
-  // 2. moments (ir.js:192-213), left-to-right sums as emitted.
-  float delta = gi[0];
-#pragma unroll
-  for (int i = 1; i < Q; ++i) delta = delta + gi[i];
-  const float rho = 1.0f + delta;
-  float m[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    float acc = 0.0f;
-    bool first = true;
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      const int ca = a == 0 ? cx(i) : (a == 1 ? cy(i) : cz(i));
-      if (ca == 0) continue;
-      if (first) {
-        acc = ca > 0 ? gi[i] : -gi[i];
-        first = false;
-      } else {
-        acc = ca > 0 ? acc + gi[i] : acc - gi[i];
-      }
-    }
-    m[a] = acc;
-  }
-  const float ux = m[0] / rho, uy = m[1] / rho, uz = m[2] / rho;
-  const float uu = ux * ux + uy * uy + uz * uz;
-
+  // 2. moments, then the branch chain ESPELHO_Y -> ESPELHO_Z -> ENTRADA ->
+  //    SAIDA -> fluid (equilibrium, Smagorinsky, TRT, sponge).
+  const Moments M = moments(gi);
   float d[Q];
-  if (ct == ESPELHO_Y) {
-    // free-slip mirror (ir.js:383-390): reflect the y component
-#pragma unroll
-    for (int i = 0; i < Q; ++i) d[i] = gi[mirror(i, 1)];
-  } else if (ct == ESPELHO_Z) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) d[i] = gi[mirror(i, 2)];
-  } else if (ct == ENTRADA) {
-    // inlet (ir.js:405-422) with the 1/7-power profile (wgsl.js:217-220):
-    // equilibrium at the prescribed velocity, delta = 0, rho = 1.
-    const float zc = (float)c.z + 0.5f;
-    const float blf = zc < p.bl ? powf(zc / p.bl_m, 0.1428571429f) : 1.0f;
-    const float ex = p.inlet[0] * blf, ey = p.inlet[1] * blf, ez = p.inlet[2] * blf;
-    const float uuin = ex * ex + ey * ey + ez * ez;
-    d[0] = wf(0) * (0.0f - 1.5f * 1.0f * uuin);
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      const float ci = dotc(i, ex, ey, ez);
-      d[i] = wf(i) * (0.0f + 1.0f * (3.0f * ci + 4.5f * ci * ci - 1.5f * uuin));
-    }
-  } else if (ct == SAIDA) {
-    // outlet (ir.js:432-444): equilibrium at the local u with delta = 0
-    d[0] = wf(0) * (-1.5f * uu);
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      const float cu = dotc(i, ux, uy, uz);
-      d[i] = wf(i) * (3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
-    }
-  } else {
-    // fluid: equilibrium (ir.js:216-232)
+  if (!boundary_cell(ct, gi, M, c.z, p, d)) {
     float e[Q];
-    e[0] = wf(0) * (delta - 1.5f * rho * uu);
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      const float cu = dotc(i, ux, uy, uz);
-      e[i] = wf(i) * (delta + rho * (3.0f * cu + 4.5f * cu * cu - 1.5f * uu));
-    }
-    // closed-form Smagorinsky with Lambda held fixed (ir.js:262-301)
+    equilibrium(M, e);
     float pab[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const int a = k < 3 ? k : (k == 5 ? 1 : 0);
-      const int b = k < 3 ? k : (k == 3 ? 1 : 2);
-      float acc = 0.0f;
-      bool first = true;
-#pragma unroll
-      for (int i = 0; i < Q; ++i) {
-        const int ca = a == 0 ? cx(i) : (a == 1 ? cy(i) : cz(i));
-        const int cb = b == 0 ? cx(i) : (b == 1 ? cy(i) : cz(i));
-        const int coef = ca * cb;
-        if (coef == 0) continue;
-        const float n = gi[i] - e[i];
-        if (first) {
-          acc = coef > 0 ? n : -n;
-          first = false;
-        } else {
-          acc = coef > 0 ? acc + n : acc - n;
-        }
-      }
-      pab[k] = acc;
-    }
-    const float pp = pab[0] * pab[0] + pab[1] * pab[1] + pab[2] * pab[2] +
-                     2.0f * (pab[3] * pab[3] + pab[4] * pab[4] + pab[5] * pab[5]);
-    const float qs = sqrtf(2.0f * pp);
-    const float tauT = 0.5f * (p.tau0 + sqrtf(p.tau0sq + p.c18cs2 * qs / rho));
-    const float omP = 1.0f / tauT;
-    const float omM = 1.0f / (p.magic / (1.0f / omP - 0.5f) + 0.5f);
-    // TRT collision per opposite pair (ir.js:304-323)
-    d[0] = gi[0] - omP * (gi[0] - e[0]);
-#pragma unroll
-    for (int i = 1; i < Q; i += 2) {
-      const int j = i + 1;
-      const float pe = 0.5f * ((gi[i] + gi[j]) - (e[i] + e[j]));
-      const float mo = 0.5f * ((gi[i] - gi[j]) - (e[i] - e[j]));
-      d[i] = gi[i] - omP * pe - omM * mo;
-      d[j] = gi[j] - omP * pe + omM * mo;
-    }
-    // sponge (ir.js:458-478): mix(dst, teq, sigma^2).  For x <= start
-    // sigma^2 = 0 and mix returns dst unchanged, so the blend is skipped.
-    const float dxs = (float)c.x - p.sponge_start;
-    if (dxs > 0.0f) {
-      const float sigma = fminf(dxs / p.sponge_len_m, 1.0f);
-      const float sig2 = sigma * sigma;
-#pragma unroll
-      for (int i = 0; i < Q; ++i) d[i] = d[i] * (1.0f - sig2) + p.tinf[i] * sig2;
-    }
+    noneq_stress(gi, e, pab);
+    trt(gi, e, smagorinsky_tau(stress_norm(pab), M.rho, p), p, d);
+    sponge(c.x, p, d);
   }
 
 #pragma unroll
   for (int i = 0; i < Q; ++i) S::st(pop_at<LB>(dst, i, cell, g), d[i]);
-  if (MACROS) macros[cell] = make_float4(ux, uy, uz, delta);
+  if (MACROS) macros[cell] = make_float4(M.ux, M.uy, M.uz, M.delta);
 }
 
 template <class S, bool MACROS, int BLOCK, int LB>
@@ -747,18 +492,6 @@ static inline dim3 cell_grid(const Geom& g) { return dim3(g.per_xcd * kXcds); }
 
 // Instantiates the statement block passed as the variadic argument with
 // S = storage trait and L = layout exponent.
-#define CFD_DISPATCH(sto, g, ...)                                          \
-  do {                                                                     \
-    if ((sto) == Store::F32) {                                             \
-      using S = StoreF32;                                                  \
-      if ((g).lb) { constexpr int L = kBlockLog2; __VA_ARGS__ }            \
-      else { constexpr int L = 0; __VA_ARGS__ }                            \
-    } else {                                                               \
-      using S = StoreF16;                                                  \
-      if ((g).lb) { constexpr int L = kBlockLog2; __VA_ARGS__ }            \
-      else { constexpr int L = 0; __VA_ARGS__ }                            \
-    }                                                                      \
-  } while (0)
 
 hipError_t launch_prep_types(const uint32_t* t32, uint8_t* t8, const Geom& g, hipStream_t st) {
   hipLaunchKernelGGL(k_prep_types, cell_grid(g), dim3(kBlock), 0, st, t32, t8, g);

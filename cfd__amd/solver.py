@@ -201,6 +201,23 @@ class SolverMI355X:
         """solver.js:511-520"""
         check(lib.cfd_update_macros(self._h))
 
+    # ------------------------------------------------------------ closure
+    def definirFechamento(self, fech) -> None:
+        """Learned eddy viscosity in every following step (None: Smagorinsky)."""
+        check(lib.cfd_set_closure(self._h, C.byref(fech.c_struct()) if fech is not None
+                                  else None))
+        self.fechamento = fech
+
+    def definirDistanciaParede(self, d: np.ndarray) -> None:
+        a = np.ascontiguousarray(d, dtype=np.float32).reshape(-1)
+        check(lib.cfd_set_wall_distance(self._h, ptr(a), a.size))
+
+    def fallbacksFechamento(self, zerar: bool = True) -> int:
+        """Fluid cell-steps that kept the Smagorinsky tau since the last reset."""
+        n = C.c_int64()
+        check(lib.cfd_closure_fallbacks(self._h, C.byref(n), 1 if zerar else 0))
+        return n.value
+
     def forcaBruta(self) -> np.ndarray:
         f = (C.c_double * 3)()
         check(lib.cfd_forces(self._h, f))

@@ -162,6 +162,59 @@ int cfd_bytes(cfd_solver *s, int64_t *bytes_out);
 /* SolverWebGPU.destruir (solver.js:523-527). */
 void cfd_destroy(cfd_solver *s);
 
+/* ---- learned sub-grid closure ----
+ * No reference counterpart: the reference's only closure is the closed-form
+ * Smagorinsky of ir.js:262-301.  A per-cell MLP 8 -> 32 -> 32 -> 1 (SiLU,
+ * SiLU, softplus) gives the eddy viscosity nu_t; the step then relaxes with
+ * tau = 1/omega_plus + 3 nu_t instead of the Smagorinsky tau.  Features of a
+ * fluid cell: Pi_xx, Pi_yy, Pi_zz, Pi_xy, Pi_xz, Pi_yz of the non-equilibrium
+ * stress divided by rho, the strain rate |S| = 3 Q_s / (2 rho tau_s) implied
+ * by the Smagorinsky solution tau_s, and the wall distance in cells.  A cell
+ * whose normalized features leave [in_lo, in_hi], or whose output is not
+ * finite, keeps the Smagorinsky tau (counted by cfd_closure_fallbacks).
+ * Inference uses bf16 operands with float32 accumulation on the matrix
+ * cores; training (cfd_mlp_train) is float32 Adam on minibatches. */
+typedef struct cfd_mlp {   /* row-major [out][in]; 1377 floats in this order */
+  float w1[32 * 8];
+  float b1[32];
+  float w2[32 * 32];
+  float b2[32];
+  float w3[32];
+  float b3;
+} cfd_mlp;
+
+typedef struct cfd_closure {
+  cfd_mlp mlp;
+  float in_shift[8], in_scale[8]; /* x_n = (x - shift) * scale                  */
+  float in_lo[8], in_hi[8];       /* Smagorinsky fallback outside [lo, hi] (x_n) */
+  float out_scale;                /* nu_t = out_scale * softplus(mlp(x_n))        */
+} cfd_closure;
+
+typedef struct cfd_train_cfg {
+  int32_t steps;                  /* Adam steps                                  */
+  int32_t batch;                  /* samples per step, a multiple of 64          */
+  float lr, beta1, beta2, eps;
+  uint64_t seed;                  /* minibatch draw                              */
+} cfd_train_cfg;
+
+/* Use the closure in every following step (NULL: back to Smagorinsky). */
+int cfd_set_closure(cfd_solver *s, const cfd_closure *c);
+/* Distance of every cell to the nearest wall cell, in cells (n = N). */
+int cfd_set_wall_distance(cfd_solver *s, const float *d, size_t n);
+/* Fluid cell-steps that fell back to Smagorinsky since the last reset. */
+int cfd_closure_fallbacks(cfd_solver *s, int64_t *count, int reset);
+/* Train params in place on X (n x 8, normalized features) and y (n targets
+ * of softplus(mlp(x)), i.e. nu_t / out_scale); loss_curve: steps floats (the
+ * minibatch mean squared error), nullable. */
+int cfd_mlp_train(const float *X, const float *y, size_t n, const cfd_train_cfg *cfg,
+                  cfd_mlp *params, float *loss_curve);
+/* Full-batch gradient of the mean squared error of softplus(mlp(X)) - y. */
+int cfd_mlp_grad(const float *X, const float *y, size_t n, const cfd_mlp *params, cfd_mlp *grad,
+                 double *loss);
+/* The closure's inference path on raw features X (n x 8): nu_t and whether the
+ * normalized features lie inside the trained range. */
+int cfd_mlp_infer(const float *X, size_t n, const cfd_closure *c, float *nu, uint8_t *inside);
+
 /* ---- batched tunnels: many independent float32 tunnels, one launch ----
  * No reference counterpart (the reference runs one tunnel per page).  Each
  * tunnel is a grid, a cell-type field and a parameter set; all tunnels share

@@ -32,6 +32,10 @@ def _load():
         getattr(lib, f"orc_forces_{suf}").argtypes = [C.c_int] * 3 + [P, P, P]
         for f in ("step", "init", "macros", "forces"):
             getattr(lib, f"orc_{f}_{suf}").restype = None
+    lib.orc_step_closure_f32.argtypes = [C.c_int] * 3 + [P, P, P, P, P, P, P]
+    lib.orc_step_closure_f32.restype = None
+    lib.orc_closure_eval.argtypes = [P, P, C.c_size_t, P]
+    lib.orc_closure_eval.restype = None
     lib.orc_voxelize.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_int, C.c_int, C.c_int,
                                  C.c_double, C.c_int, P, P, P]
     lib.orc_voxelize.restype = C.c_int
@@ -145,3 +149,35 @@ def voxelize(pos, idx, grid, banda=1.5, selar=True):
 
 def threads() -> int:
     return lib().orc_threads()
+
+
+def closure_array(params, shift, scale, lo, hi, out_scale):
+    """The float32 block orc_closure reads: 1377 MLP parameters (cfd_mlp
+    order), then shift, scale, lo, hi (8 each) and out_scale."""
+    return np.ascontiguousarray(np.concatenate([
+        np.asarray(params, np.float32).reshape(-1), np.asarray(shift, np.float32),
+        np.asarray(scale, np.float32), np.asarray(lo, np.float32), np.asarray(hi, np.float32),
+        np.array([out_scale], np.float32)]), dtype=np.float32)
+
+
+def closure_eval(cl, X):
+    """nu of the learned closure for raw features X (n, 8); NaN outside the range."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    out = np.empty(X.shape[0], dtype=np.float32)
+    lib().orc_closure_eval(_p(cl), _p(X), X.shape[0], _p(out))
+    return out
+
+
+def step_closure(grid, src, tipo, params, cl, wdist, macros=False):
+    """One float32 oracle step with the learned closure (cl from
+    closure_array, wdist: float32 per cell)."""
+    nx, ny, nz = grid
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    dst = src.copy()
+    t = np.ascontiguousarray(tipo, dtype=np.uint32)
+    m = np.zeros(4 * nx * ny * nz, dtype=np.float32) if macros else None
+    wd = np.ascontiguousarray(wdist, dtype=np.float32)
+    lib().orc_step_closure_f32(nx, ny, nz, _p(src.reshape(-1)), _p(dst.reshape(-1)), _p(t),
+                               _p(np.ascontiguousarray(params, dtype=np.float64)),
+                               _p(m) if m is not None else None, _p(cl), _p(wd))
+    return dst, m

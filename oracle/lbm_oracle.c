@@ -72,6 +72,61 @@ static void tables(void) {
   done = 1;
 }
 
+/* ---- learned closure (no reference counterpart) ----
+ * The MLP 8 -> 32 -> 32 -> 1 of the library's closure, in float32 with the
+ * bf16 rounding of its matrix-core path made explicit: the normalized
+ * features, the weights and each hidden activation are rounded to bf16
+ * (round to nearest even); products accumulate in float32 from the bias in
+ * increasing input order (the matrix cores sum in another order, so results
+ * agree to float32 accumulation error, not bit for bit).  Returns
+ * out_scale * softplus(z), or NaN when a normalized feature leaves
+ * [lo, hi] (the caller then keeps the Smagorinsky tau). */
+typedef struct {
+  float p[1377];                  /* w1[32][8] b1[32] w2[32][32] b2[32] w3[32] b3 */
+  float shift[8], scale[8], lo[8], hi[8], out_scale;
+} orc_closure;
+
+static float bf16r(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return f;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+static float silu_ref(float a) { return a * (1.0f / (1.0f + expf(-a))); }
+
+static float orc_closure_nu(const orc_closure* c, const float x[8]) {
+  const float *w1 = c->p, *b1 = c->p + 256, *w2 = c->p + 288, *b2 = c->p + 1312,
+              *w3 = c->p + 1344, b3 = c->p[1376];
+  float xb[8], h1[32], h2[32];
+  for (int k = 0; k < 8; ++k) {
+    const float xn = (x[k] - c->shift[k]) * c->scale[k];
+    if (!(xn >= c->lo[k] && xn <= c->hi[k])) return NAN;
+    xb[k] = bf16r(xn);
+  }
+  for (int o = 0; o < 32; ++o) {
+    float acc = b1[o];
+    for (int k = 0; k < 8; ++k) acc += bf16r(w1[o * 8 + k]) * xb[k];
+    h1[o] = bf16r(silu_ref(acc));
+  }
+  for (int o = 0; o < 32; ++o) {
+    float acc = b2[o];
+    for (int k = 0; k < 32; ++k) acc += bf16r(w2[o * 32 + k]) * h1[k];
+    h2[o] = bf16r(silu_ref(acc));
+  }
+  float z = b3;
+  for (int k = 0; k < 32; ++k) z += bf16r(w3[k]) * h2[k];
+  const float sp = z > 20.0f ? z : (z < -15.0f ? expf(z) : log1pf(expf(z)));
+  return c->out_scale * sp;
+}
+
+void orc_closure_eval(const orc_closure* c, const float* X, size_t n, float* nu) {
+  for (size_t s = 0; s < n; ++s) nu[s] = orc_closure_nu(c, X + 8 * s);
+}
+
 /* ---- float32 instantiation: literals as the generator prints them ---- */
 #define R float
 #define SFX(n) n##_f32
@@ -122,6 +177,13 @@ void orc_step_f32(int nx, int ny, int nz, const float* src, float* dst, const ui
                   const double* params, float* macros);
 void orc_step_f64(int nx, int ny, int nz, const double* src, double* dst, const uint32_t* tipo,
                   const double* params, double* macros);
+void orc_step_closure_f32(int nx, int ny, int nz, const float* src, float* dst,
+                          const uint32_t* tipo, const double* params, float* macros,
+                          const orc_closure* cl, const float* wdist) {
+  tables();
+  step_impl_f32(nx, ny, nz, src, dst, tipo, params, macros, cl, wdist);
+}
+
 int orc_setup(void) {
   tables();
   return Q;

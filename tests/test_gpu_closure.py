@@ -1,0 +1,221 @@
+"""Learned closure on the GPU: the matrix-core MLP, the hand-written
+training kernels and the closure step.
+
+* inference (bf16 operands, float32 accumulation) against the oracle's
+  bf16-rounding float32 MLP, and against a plain float32 torch MLP;
+* the full-batch gradient of the training kernels against torch autograd
+  (float32) and a short training run that must fit a teacher network;
+* the closure step, bit for bit, against the plain step where the closure
+  must reduce to it (every cell out of range -> Smagorinsky; nu_t = 0 ->
+  Cs = 0), and against the oracle's closure step with a random MLP.
+"""
+import numpy as np
+import pytest
+
+from cfd__amd import tunel
+from cfd__amd.fechamento import Fechamento, gradiente, iniciar_mlp, treinar_mlp, N_PARAMS
+
+pytestmark = pytest.mark.gpu
+
+
+def _fech(seed=0, out_scale=2e-3, spread=1.0):
+    rng = np.random.default_rng(seed)
+    p = iniciar_mlp(seed, bias_saida=0.3)
+    p[1344:1376] *= 10.0   # a non-trivial output layer
+    shift = rng.normal(0, 1e-3, 8).astype(np.float32)
+    scale = (1.0 / rng.uniform(5e-4, 2e-3, 8)).astype(np.float32) * spread
+    shift[7], scale[7] = 5.0, 0.1 * spread       # wall distance, cells
+    lo = np.full(8, -60, np.float32)
+    hi = np.full(8, 60, np.float32)
+    return Fechamento(p, shift, scale, lo, hi, out_scale)
+
+
+def _torch_mlp(p, xn, bf16=False):
+    torch = pytest.importorskip("torch")
+    P = torch.tensor(np.asarray(p, np.float32))
+    r = (lambda t: t.to(torch.bfloat16).to(torch.float32)) if bf16 else (lambda t: t)
+    w1 = r(P[0:256].reshape(32, 8)); b1 = P[256:288]
+    w2 = r(P[288:1312].reshape(32, 32)); b2 = P[1312:1344]
+    w3 = r(P[1344:1376]); b3 = P[1376]
+    x = r(torch.tensor(xn, dtype=torch.float32))
+    h1 = r(torch.nn.functional.silu(x @ w1.T + b1))
+    h2 = r(torch.nn.functional.silu(h1 @ w2.T + b2))
+    z = h2 @ w3 + b3
+    return torch.nn.functional.softplus(z).numpy()
+
+
+def test_mlp_inference_matches_oracle_and_torch(orc):
+    fech = _fech(1)
+    rng = np.random.default_rng(2)
+    n = 4096 + 37                       # a ragged last wave
+    xn = rng.normal(0, 1.5, (n, 8)).astype(np.float32)
+    X = (xn / fech.scale + fech.shift).astype(np.float32)
+    X[5, 3] = 1e3                       # out of the trained range
+    nu, dentro = fech.inferir(X)
+    assert not dentro[5] and dentro.sum() == n - 1
+    cl = orc.closure_array(fech.params, fech.shift, fech.scale, fech.lo, fech.hi, fech.out_scale)
+    ref = orc.closure_eval(cl, X)
+    ok = np.isfinite(ref)
+    assert ok.sum() == n - 1
+    scale = np.abs(ref[ok]).max()
+    err = np.abs(nu[ok] - ref[ok]) / scale
+    # same bf16 roundings; float32 sums in another order and the fast exp:
+    # most rows agree to float32 noise, a bf16 rounding that flips moves a row
+    # by at most a bf16 step of one hidden unit
+    assert np.median(err) < 1e-5 and err.max() < 5e-3, (np.median(err), err.max())
+    # plain float32 torch MLP: the bf16 operand rounding is the only difference
+    t32 = fech.out_scale * _torch_mlp(fech.params, fech.normalizar(X)[ok])
+    assert np.abs(nu[ok] - t32).max() / scale < 5e-2
+    tbf = fech.out_scale * _torch_mlp(fech.params, fech.normalizar(X)[ok], bf16=True)
+    assert np.median(np.abs(nu[ok] - tbf) / scale) < 1e-5
+
+
+def test_training_gradient_matches_torch_autograd():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(3)
+    n = 1000
+    Xn = rng.normal(0, 1, (n, 8)).astype(np.float32)
+    y = rng.uniform(0, 2, n).astype(np.float32)
+    p = iniciar_mlp(4, bias_saida=0.1)
+    g, loss = gradiente(Xn, y, p)
+    P = torch.tensor(p, requires_grad=True)
+    x = torch.tensor(Xn)
+    h1 = torch.nn.functional.silu(x @ P[0:256].reshape(32, 8).T + P[256:288])
+    h2 = torch.nn.functional.silu(h1 @ P[288:1312].reshape(32, 32).T + P[1312:1344])
+    yhat = torch.nn.functional.softplus(h2 @ P[1344:1376] + P[1376])
+    L = ((yhat - torch.tensor(y)) ** 2).mean()
+    L.backward()
+    gt = P.grad.numpy()
+    assert abs(loss - L.item()) <= 1e-5 * L.item()
+    assert np.abs(g - gt).max() <= 1e-4 * np.abs(gt).max()
+    # every block of the parameter vector is exercised
+    for a, b in ((0, 256), (256, 288), (288, 1312), (1312, 1344), (1344, 1376), (1376, 1377)):
+        assert np.abs(gt[a:b]).max() > 0
+
+
+def test_training_fits_a_teacher_network():
+    rng = np.random.default_rng(5)
+    n = 20000
+    Xn = rng.normal(0, 1, (n, 8)).astype(np.float32)
+    teacher = iniciar_mlp(11, bias_saida=0.5)
+    y = _torch_mlp(teacher, Xn).astype(np.float32)
+    p0 = iniciar_mlp(12)
+    _, l0 = gradiente(Xn, y, p0)
+    p, curva = treinar_mlp(Xn, y, p0, passos=3000, lote=1024, lr=1e-3, seed=7)
+    _, l1 = gradiente(Xn, y, p)
+    assert curva.shape == (3000,) and np.isfinite(curva).all()
+    assert l1 < 0.03 * l0, (l0, l1)
+    assert curva[-100:].mean() < 0.1 * curva[:100].mean()
+    # deterministic: the same seed gives the same parameters
+    p2, _ = treinar_mlp(Xn, y, p0, passos=50, lote=1024, lr=1e-3, seed=7)
+    p3, _ = treinar_mlp(Xn, y, p0, passos=50, lote=1024, lr=1e-3, seed=7)
+    assert np.array_equal(p2, p3)
+
+
+def _tunnel(grid=(40, 24, 20)):
+    nx, ny, nz = grid
+    g = {"nx": nx, "ny": ny, "nz": nz}
+    corpo = tunel.corpoAnalitico(g, {"tipo": "esfera", "cx": nx * 0.35, "cy": ny * 0.5,
+                                     "cz": nz * 0.45, "raio": min(ny, nz) * 0.2})
+    tipo, esponja, _ = tunel.montarTipos(g, corpo, esteira=True)
+    return g, tipo, esponja
+
+
+def _state(g, seed, amp=0.02):
+    from cfd__amd import lattice
+    nx, ny, nz = g["nx"], g["ny"], g["nz"]
+    rng = np.random.default_rng(seed)
+    Z, Y, X = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    u = [0.05 + amp * np.sin(2 * np.pi * X / nx) * np.cos(2 * np.pi * Z / nz),
+         amp * np.cos(2 * np.pi * Y / ny), amp * np.sin(2 * np.pi * (X + Y) / nx)]
+    f = lattice.equilibrium(1e-3 * np.sin(2 * np.pi * Z / nz), u).reshape(19, -1)
+    f = f + 3e-4 * rng.standard_normal(f.shape) * lattice.W[:, None]
+    return np.ascontiguousarray(f, dtype=np.float32)
+
+
+def _run(g, tipo, esponja, g0, n, fech=None, wd=None, precisao="fp32", lesCs=0.1):
+    from cfd__amd.solver import SolverMI355X
+    s = SolverMI355X.criar(g, precisao=precisao)
+    try:
+        s.definirTipos(tipo)
+        s.configurar(omegaPlus=1.8, lesCs=lesCs, inletU=[0.05, 0, 0], beltU=[0.05, 0, 0],
+                     esponja=esponja, espessuraCL=4.0)
+        s.frente = "A"
+        s.definirPopulacoes(g0)
+        if fech is not None:
+            s.definirDistanciaParede(wd)
+            s.definirFechamento(fech)
+        s.passo(n)
+        out = s.lerPopulacoes()
+        fb = s.fallbacksFechamento() if fech is not None else 0
+    finally:
+        s.destruir()
+    return out, fb
+
+
+@pytest.mark.parametrize("precisao", ["fp32", "fp16"])
+def test_closure_out_of_range_is_smagorinsky_bit_for_bit(precisao):
+    from cfd__amd.fechamento import distancia_parede
+    g, tipo, esp = _tunnel()
+    g0 = _state(g, 1)
+    fech = _fech(2)
+    fech.lo[:] = 1e30
+    fech.hi[:] = 1e30                  # no cell is inside: every fluid cell falls back
+    wd = distancia_parede(tipo, g)
+    a, fb = _run(g, tipo, esp, g0, 7, fech, wd, precisao)
+    b, _ = _run(g, tipo, esp, g0, 7, None, None, precisao)
+    wall = np.isin(tipo, [1, 4, 7])
+    assert np.array_equal(a[:, ~wall], b[:, ~wall])
+    fluid = np.sum(~wall & ~np.isin(tipo, [2, 3, 5, 6]))
+    assert fb == 7 * fluid
+
+
+def test_closure_with_zero_viscosity_is_the_cs0_step_bit_for_bit():
+    from cfd__amd.fechamento import distancia_parede
+    g, tipo, esp = _tunnel()
+    g0 = _state(g, 3)
+    fech = _fech(4, out_scale=0.0)
+    fech.lo[:] = -np.inf
+    fech.hi[:] = np.inf
+    wd = distancia_parede(tipo, g)
+    a, fb = _run(g, tipo, esp, g0, 5, fech, wd, lesCs=0.0)
+    b, _ = _run(g, tipo, esp, g0, 5, None, None, lesCs=0.0)
+    wall = np.isin(tipo, [1, 4, 7])
+    assert fb == 0
+    assert np.array_equal(a[:, ~wall], b[:, ~wall])
+
+
+def test_closure_step_matches_oracle(orc):
+    from cfd__amd.fechamento import distancia_parede
+    g, tipo, esp = _tunnel()
+    grid = (g["nx"], g["ny"], g["nz"])
+    g0 = _state(g, 5)
+    fech = _fech(6, out_scale=4e-3, spread=0.5)
+    wd = distancia_parede(tipo, g)
+    wdq = np.minimum(np.floor(wd * 8 + 0.5), 255).astype(np.float32) / 8   # the device's 1/8 cell
+    par = orc.params_array(1.8, lesCs=0.1, inletU=[0.05, 0, 0], beltU=[0.05, 0, 0],
+                           esponja=esp, espessuraCL=4.0)
+    cl = orc.closure_array(fech.params, fech.shift, fech.scale, fech.lo, fech.hi, fech.out_scale)
+    ref = g0.copy()
+    wall = np.isin(tipo, [1, 4, 7])
+    done = 0
+    for target in (1, 10):
+        for _ in range(target - done):
+            ref, _ = orc.step_closure(grid, ref, tipo, par, cl, wdq)
+        done = target
+        got, fb = _run(g, tipo, esp, g0, target, fech, wd)
+        assert fb < 0.05 * target * np.sum(~wall)   # the MLP is in use almost everywhere
+        r = ref.reshape(19, -1)[:, ~wall]
+        err = np.abs(got[:, ~wall].astype(np.float64) - r).max(axis=0) / np.abs(r).max()
+        # one step: float32 agreement.  Over ten steps a hidden activation that
+        # rounds to the other bf16 neighbour (the matrix cores sum in another
+        # order) changes a cell's nu_t by up to a bf16 step and the flow
+        # carries it: a few cells move by ~1e-4, the bulk stays at float32 noise.
+        if target == 1:
+            assert err.max() <= 1e-5, err.max()
+        else:
+            assert err.max() <= 3e-4 and np.percentile(err, 99) <= 1e-5, (
+                err.max(), np.percentile(err, 99))
+    # and the closure changed the flow: not the Smagorinsky result
+    smag, _ = _run(g, tipo, esp, g0, 10, None, None)
+    assert np.abs(smag[:, ~wall] - got[:, ~wall]).max() > 1e-6

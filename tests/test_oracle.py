@@ -340,3 +340,78 @@ def test_init_and_macros_closed_form(orc):
     assert np.abs(m[:, 3] - d).max() < 1e-16
     for a in range(3):
         assert np.abs(m[:, a] - u[a]).max() < 1e-15
+
+
+# ---- learned closure in the oracle (test infrastructure for the GPU closure)
+
+def _bf16(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def _mlp_numpy(p, xn):
+    """Independent float64 evaluation of the closure MLP on bf16-rounded
+    weights, inputs and hidden activations (the oracle sums in float32)."""
+    sig = lambda a: 1.0 / (1.0 + np.exp(-a))
+    w1 = _bf16(p[0:256]).reshape(32, 8).astype(np.float64)
+    w2 = _bf16(p[288:1312]).reshape(32, 32).astype(np.float64)
+    w3 = _bf16(p[1344:1376]).astype(np.float64)
+    x = _bf16(xn).astype(np.float64)
+    a1 = x @ w1.T + p[256:288]
+    h1 = _bf16((a1 * sig(a1)).astype(np.float32)).astype(np.float64)
+    a2 = h1 @ w2.T + p[1312:1344]
+    h2 = _bf16((a2 * sig(a2)).astype(np.float32)).astype(np.float64)
+    z = h2 @ w3 + p[1376]
+    return np.logaddexp(0.0, z)
+
+
+def test_oracle_closure_mlp_against_independent_numpy(orc):
+    rng = np.random.default_rng(21)
+    p = rng.normal(0, 0.4, 1377).astype(np.float32)
+    shift = rng.normal(0, 1, 8).astype(np.float32)
+    scale = rng.uniform(0.5, 2, 8).astype(np.float32)
+    lo, hi = np.full(8, -5, np.float32), np.full(8, 5, np.float32)
+    X = rng.normal(0, 1.5, (3000, 8)).astype(np.float32)
+    cl = orc.closure_array(p, shift, scale, lo, hi, 0.7)
+    nu = orc.closure_eval(cl, X)
+    xn = ((X - shift) * scale).astype(np.float32)
+    inside = np.all((xn >= lo) & (xn <= hi), axis=1)
+    assert np.array_equal(np.isfinite(nu), inside) and 0 < inside.sum() < 3000
+    ref = 0.7 * _mlp_numpy(p, xn[inside])
+    err = np.abs(nu[inside] - ref) / np.abs(ref).max()
+    # float32 vs float64 sums: a few hidden activations land on the other
+    # bf16 neighbour; the bulk agrees to float32 precision
+    assert np.median(err) < 1e-6 and err.max() < 5e-3
+    # a dropped layer or a transposed weight matrix would not come close
+    pt = p.copy()
+    pt[288:1312] = p[288:1312].reshape(32, 32).T.reshape(-1)
+    assert np.abs(0.7 * _mlp_numpy(pt, xn[inside]) - ref).max() > 1e-2 * np.abs(ref).max()
+
+
+def test_oracle_closure_step_reduces_to_the_plain_step(orc):
+    g = {"nx": 24, "ny": 14, "nz": 12}
+    grid = (24, 14, 12)
+    corpo = tunel.corpoAnalitico(g, {"tipo": "esfera", "cx": 8, "cy": 7, "cz": 6, "raio": 3})
+    tipo, esp, _ = tunel.montarTipos(g, corpo, esteira=True)
+    rng = np.random.default_rng(3)
+    u = [0.05 + 0.01 * rng.standard_normal(24 * 14 * 12), 0.01 * rng.standard_normal(24 * 14 * 12),
+         np.zeros(24 * 14 * 12)]
+    g0 = lattice.equilibrium(np.zeros(24 * 14 * 12), u).reshape(19, -1).astype(np.float32)
+    wd = np.zeros(24 * 14 * 12, np.float32)
+    p = rng.normal(0, 0.3, 1377).astype(np.float32)
+    for cs in (0.1, 0.0):
+        par = orc.params_array(1.7, lesCs=cs, inletU=[0.05, 0, 0], beltU=[0.05, 0, 0],
+                               esponja=esp)
+        plain, _ = orc.step(grid, g0, tipo, par)
+        # nothing inside the trained range: Smagorinsky everywhere
+        out = orc.closure_array(p, np.zeros(8), np.ones(8), np.full(8, 1e30), np.full(8, 1e30), 1)
+        a, _ = orc.step_closure(grid, g0, tipo, par, out, wd)
+        assert np.array_equal(a, plain)
+        if cs == 0.0:
+            # nu_t = 0 everywhere: tau = 1/omega, the Cs = 0 step
+            zero = orc.closure_array(p, np.zeros(8), np.ones(8), np.full(8, -np.inf),
+                                     np.full(8, np.inf), 0.0)
+            b, _ = orc.step_closure(grid, g0, tipo, par, zero, wd)
+            assert np.array_equal(b, plain)
