@@ -1,0 +1,377 @@
+"""Learned-closure workload: train on filtered fine runs, judge on a coarse run.
+
+The vehicle of the 320x160x128 benchmark (meshes.vehicle, length 96) is
+scaled by 0.75 for the coarse 240x120x96 grid and by 1.5 for the fine
+480x240x192 grid (exactly 2x finer).  Acoustic scaling keeps u = 0.08 in
+both lattices, so the fine lattice viscosity is twice the coarse one; the
+coarse run uses the reference's omega cap (1.90), the fine run the same
+physical viscosity (omega 1.81), both with the plain solver's Smagorinsky
+(Cs = 0.1) unless the learned closure is on.  Floor: no-slip PAREDE with the
+app's 1/7-power inlet boundary layer of round(0.12 nz) cells.
+
+Training data (gerar_dados): for each training shape a fine float32 run;
+snapshots of the fine velocity are box-filtered 2x2x2 onto the coarse grid.
+Per coarse fluid cell: the filtered strain rate S (central differences), the
+sub-grid stress tau_ab = <u_a u_b> - <u_a><u_b>, and the eddy viscosity that
+best explains it, nu* = max(0, -tau^d : S / (2 S : S)).  The features are
+what the coarse step computes from its populations when the flow equals the
+filtered one: Pi / rho = -(2/3) tau S with tau = 1/omega + 3 nu*, the
+Smagorinsky strain estimate |S| = 3 Q_s / (2 rho tau_s), and the wall
+distance.
+
+Judging (avaliar): Cd (time average) and St (lift crossings) of the coarse
+run with the learned closure in float16 storage, against the fine run and
+against the coarse Smagorinsky run; the closure is accepted only if it is at
+least as close to the fine Cd as Smagorinsky (otherwise Smagorinsky stays).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import time
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import meshes, tunel
+from .fechamento import Fechamento, distancia_parede
+from .forces import JanelaCd, coeficientes
+from .lote_casos import strouhal_periodos
+from .voxel import voxelizar
+
+U = 0.08
+OMEGA_C = 1.90
+NU_C = (1.0 / OMEGA_C - 0.5) / 3.0
+CS = 0.1
+COARSE = (240, 120, 96)
+BASE_SCALE = 0.75            # meshes.vehicle is laid out for 320x160x128
+AVALIACAO_SEED = 7
+TREINO_SEEDS = tuple(range(11, 19))
+# the closure trained by `python -m cfd__amd.fechamento_caso` (8 shapes,
+# 20 000 Adam steps), shipped with the package
+MODELO = __import__("pathlib").Path(__file__).resolve().parent / "modelos" / "fechamento_c5.npz"
+
+
+def omega_de(nu: float) -> float:
+    return 1.0 / (3.0 * nu + 0.5)
+
+
+def geometria(seed: int, escala: float):
+    pos, idx, info = meshes.vehicle(seed=seed)
+    return (pos * np.float32(escala)).astype(np.float32), idx, info
+
+
+def montar(seed: int, fator: int):
+    """Grid, cell types, sponge, frontal area and wall distance of the
+    vehicle `seed` on the coarse grid refined `fator` times (1 or 2)."""
+    nx, ny, nz = (c * fator for c in COARSE)
+    g = {"nx": nx, "ny": ny, "nz": nz}
+    pos, idx, info = geometria(seed, BASE_SCALE * fator)
+    vox = voxelizar(pos, idx, g, banda=1.5, selarFrestas=True, comSdf=False)
+    tipo, esponja, _ = tunel.montarTipos(g, vox["tipo"], esteira=False)
+    area = tunel.areaFrontalCorpo(vox["tipo"], g)
+    return {"grade": g, "tipo": tipo, "esponja": esponja, "area": area, "fator": fator,
+            "seed": seed, "parede": distancia_parede(tipo, g)}
+
+
+def _units():
+    return SimpleNamespace(uLb=U)
+
+
+def rodar(caso: dict, passos: int, precisao: str = "fp32", fech: Fechamento | None = None,
+          amostra: int = 100, media_desde: int | None = None, instantaneos=(), ao_instantaneo=None,
+          log=print) -> dict:
+    """One tunnel run: ramp over 1000 coarse-equivalent steps, then `passos`
+    steps; forces every `amostra` steps; Cd / Cl / St over steps >=
+    media_desde; `ao_instantaneo(solver, k)` at the listed steps."""
+    from .solver import SolverMI355X
+
+    g, fator = caso["grade"], caso["fator"]
+    nu = NU_C * fator
+    s = SolverMI355X.criar(g, precisao=precisao)
+    cfg = dict(omegaPlus=omega_de(nu), lesCs=CS, esponja=caso["esponja"],
+               beltU=[0.0, 0.0, 0.0], espessuraCL=float(math.floor(g["nz"] * 0.12 + 0.5)))
+    try:
+        s.definirTipos(caso["tipo"])
+        s.configurar(inletU=[U, 0.0, 0.0], **cfg)
+        s.calibrarRepouso()
+        s.rampa(1000 * fator, lambda f: s.configurar(inletU=[U * f, 0.0, 0.0], **cfg))
+        s.inicializar()
+        if fech is not None:
+            s.definirDistanciaParede(caso["parede"])
+            s.definirFechamento(fech)
+        media_desde = passos // 2 if media_desde is None else media_desde
+        cd, cl = JanelaCd(10 ** 7), JanelaCd(10 ** 7)
+        serie_cl, serie_cs = [], []
+        pendentes = sorted(k for k in set(instantaneos) if 0 < k <= passos)
+        kernel_ms = 0.0
+        k = 0
+        while k < passos:
+            n = min(amostra, passos - k)
+            if pendentes:
+                n = min(n, pendentes[0] - k)
+            kernel_ms += s.passoCronometrado(n)
+            k += n
+            if pendentes and k == pendentes[0]:
+                ao_instantaneo(s, k)
+                pendentes.pop(0)
+            if k % amostra == 0 and k >= media_desde:
+                co = coeficientes(s.medirForcas(), _units(), caso["area"], g,
+                                  corrigirBloqueio=True, theta=2.5)
+                cd.adicionar(co["cd"])
+                cl.adicionar(co["cl"])
+                serie_cl.append(co["cl"])
+                serie_cs.append(co["cs"])
+        fallbacks = s.fallbacksFechamento() if fech is not None else 0
+        massa, nao_finitos = s.massa()
+    finally:
+        s.destruir()
+    N = g["nx"] * g["ny"] * g["nz"]
+    # lift frequency per step of this lattice (D = u = 1 in strouhal_periodos),
+    # then St on the coarse body length L = 72 cells: f_coarse * L / u, with
+    # one coarse step = `fator` steps of this lattice
+    comp = fator * 96.0 * BASE_SCALE / U
+    st = {nome: (strouhal_periodos(serie, amostra, 1.0, 1.0) * comp if len(serie) >= 32
+                 else math.nan) for nome, serie in (("sustentacao", serie_cl),
+                                                    ("lateral", serie_cs))}
+    fluid = int(np.sum(np.asarray(caso["tipo"]) == 0))
+    return {"cd": cd.media, "cd_std": cd.desvio, "cl": cl.media, "cl_std": cl.desvio,
+            "st_comprimento": st, "amostras": cd.n, "kernel_ms": kernel_ms,
+            "mlups": N * passos / (kernel_ms * 1e-3) / 1e6, "nao_finitos": nao_finitos,
+            "fallback_frac": fallbacks / max(1, fluid * passos) if fech is not None else None,
+            "passos": passos, "precisao": precisao, "fechamento": fech is not None}
+
+
+# ------------------------------------------------------------------ data
+
+def _box2(a: np.ndarray) -> np.ndarray:
+    """2x2x2 mean of a (2nz, 2ny, 2nx, ...) array."""
+    nz, ny, nx = a.shape[0] // 2, a.shape[1] // 2, a.shape[2] // 2
+    return a.reshape(nz, 2, ny, 2, nx, 2, *a.shape[3:]).mean(axis=(1, 3, 5))
+
+
+def filtrar(macros_f: np.ndarray, fino: dict, grosso: dict, rng, por_instantaneo: int):
+    """Features X (n, 8) and targets nu* (n) from one fine snapshot."""
+    gf, gc = fino["grade"], grosso["grade"]
+    nzf, nyf, nxf = gf["nz"], gf["ny"], gf["nx"]
+    m = macros_f.reshape(nzf, nyf, nxf, 4).astype(np.float64)
+    u = m[..., :3]
+    ub = _box2(u)                                            # (nz, ny, nx, 3)
+    uu = np.empty(ub.shape[:3] + (6,))
+    pares = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+    for k, (a, b) in enumerate(pares):
+        uu[..., k] = _box2(u[..., a] * u[..., b]) - ub[..., a] * ub[..., b]
+    rho = 1.0 + _box2(m[..., 3])
+    # valid coarse cells: fluid on the coarse grid, all 8 fine children
+    # non-wall, all 6 neighbours valid too (central differences)
+    tf = np.asarray(fino["tipo"]).reshape(nzf, nyf, nxf)
+    filhos_ok = _box2((~np.isin(tf, (1, 4, 7))).astype(np.float64)) == 1.0
+    tc = np.asarray(grosso["tipo"]).reshape(gc["nz"], gc["ny"], gc["nx"])
+    ok = filhos_ok & (tc == 0)
+    viz = ok.copy()
+    for ax in range(3):
+        viz[(slice(None),) * ax + (slice(1, -1),)] &= (
+            ok[(slice(None),) * ax + (slice(2, None),)] & ok[(slice(None),) * ax + (slice(0, -2),)])
+        idx0 = [slice(None)] * 3
+        idx0[ax] = 0
+        viz[tuple(idx0)] = False
+        idx0[ax] = -1
+        viz[tuple(idx0)] = False
+    ok = viz
+    # strain rate of the filtered field (coarse lattice units)
+    grad = np.zeros(ub.shape[:3] + (3, 3))                   # d u_a / d x_b
+    for b, ax in ((0, 2), (1, 1), (2, 0)):
+        grad[..., :, b] = 0.5 * (np.roll(ub, -1, axis=ax) - np.roll(ub, 1, axis=ax))
+    S = 0.5 * (grad + np.swapaxes(grad, -1, -2))
+    Sv = np.stack([S[..., a, b] for a, b in pares], axis=-1)  # (..., 6)
+    w = np.array([1, 1, 1, 2, 2, 2], dtype=np.float64)       # off-diagonals twice
+    SS = (Sv * Sv * w).sum(-1)
+    trace = (uu[..., 0] + uu[..., 1] + uu[..., 2]) / 3.0
+    tdev = uu.copy()
+    tdev[..., :3] -= trace[..., None]
+    nu_star = np.maximum(0.0, -(tdev * Sv * w).sum(-1) / np.maximum(2.0 * SS, 1e-30))
+    smag = np.sqrt(2.0 * SS)
+    ok &= smag > 1e-7
+    idx = np.flatnonzero(ok.reshape(-1))
+    if idx.size == 0:
+        return np.zeros((0, 8), np.float32), np.zeros(0, np.float32)
+    # half uniform, half the highest strain rates
+    k = min(por_instantaneo // 2, idx.size)
+    uni = rng.choice(idx, size=k, replace=False)
+    top = idx[np.argsort(smag.reshape(-1)[idx])[-k:]]
+    sel = np.unique(np.concatenate([uni, top]))
+    Svs = Sv.reshape(-1, 6)[sel]
+    nus = nu_star.reshape(-1)[sel]
+    rhos = rho.reshape(-1)[sel]
+    tau0 = 1.0 / OMEGA_C
+    tau = tau0 + 3.0 * nus
+    pi_rho = -(2.0 / 3.0) * tau[:, None] * Svs               # Pi / rho
+    pi = pi_rho * rhos[:, None]
+    qs = np.sqrt(2.0 * (pi * pi * w).sum(-1))
+    tau_s = 0.5 * (tau0 + np.sqrt(tau0 * tau0 + 18.0 * CS * CS * qs / rhos))
+    X = np.empty((sel.size, 8), dtype=np.float32)
+    X[:, :6] = pi_rho
+    X[:, 6] = 1.5 * qs / (rhos * tau_s)
+    X[:, 7] = np.asarray(grosso["parede"]).reshape(-1)[sel]
+    return X, nus.astype(np.float32)
+
+
+def gerar_dados(seeds=TREINO_SEEDS, passos_finos: int = 12000, instantaneos: int = 5,
+                intervalo: int = 1000, por_instantaneo: int = 150000, seed: int = 0, log=print):
+    rng = np.random.default_rng(seed)
+    Xs, ys, info = [], [], []
+    for sd in seeds:
+        t0 = time.perf_counter()
+        fino, grosso = montar(sd, 2), montar(sd, 1)
+        marcas = [passos_finos - intervalo * k for k in range(instantaneos)
+                  if passos_finos - intervalo * k > passos_finos // 2]
+
+        def colher(s, k):
+            s.atualizarMacros()
+            X, y = filtrar(s.lerMacros(), fino, grosso, rng, por_instantaneo)
+            Xs.append(X)
+            ys.append(y)
+
+        r = rodar(fino, passos_finos, "fp32", instantaneos=marcas, ao_instantaneo=colher,
+                  amostra=200, media_desde=passos_finos // 2, log=log)
+        info.append({"seed": sd, "cd_fino": r["cd"], "mlups": r["mlups"],
+                     "segundos": time.perf_counter() - t0})
+        log(f"[dados] shape {sd}: Cd {r['cd']:.4f}, {sum(x.shape[0] for x in Xs)} samples, "
+            f"{time.perf_counter() - t0:.1f} s")
+    return np.concatenate(Xs), np.concatenate(ys), info
+
+
+# ------------------------------------------------------------------ calibration
+
+def calibrar(fech: Fechamento, seeds=TREINO_SEEDS[:3], fatores=(1.0, 10.0, 100.0, 1000.0),
+             passos_grossos: int = 9000, log=print) -> dict:
+    """Solver in the loop: scale the trained output by each factor, run the
+    coarse float16 tunnel of each calibration shape and keep the factor whose
+    time-averaged Cd is closest to the fine runs' (mean relative error).  The
+    a-priori target only sees the sub-grid stress of the filtered field; the
+    coarse run also needs the dissipation that its own discretization lacks,
+    which only a coarse run can measure."""
+    finos, grossos = {}, {}
+    for sd in seeds:
+        grossos[sd] = montar(sd, 1)
+        finos[sd] = rodar(montar(sd, 2), 2 * passos_grossos, "fp32", amostra=200,
+                          media_desde=2 * passos_grossos // 3)["cd"]
+    medidas = []
+    base = fech.out_scale
+    for fa in fatores:
+        fech.out_scale = base * fa
+        cds = [rodar(grossos[sd], passos_grossos, "fp16", fech=fech,
+                     media_desde=passos_grossos // 3)["cd"] for sd in seeds]
+        err = float(np.mean([abs(c - finos[sd]) / finos[sd] for c, sd in zip(cds, seeds)]))
+        medidas.append({"fator": fa, "cd": cds, "erro_medio": err})
+        log(f"[calibrar] factor {fa:g}: mean |Cd error| {err:.4f}")
+    smag = [rodar(grossos[sd], passos_grossos, "fp16", media_desde=passos_grossos // 3)["cd"]
+            for sd in seeds]
+    err_smag = float(np.mean([abs(c - finos[sd]) / finos[sd] for c, sd in zip(smag, seeds)]))
+    melhor = min(medidas, key=lambda m: m["erro_medio"])
+    fech.out_scale = base * melhor["fator"]
+    log(f"[calibrar] chosen factor {melhor['fator']:g} (Smagorinsky: {err_smag:.4f})")
+    return {"finos": finos, "medidas": medidas, "smagorinsky": {"cd": smag, "erro_medio": err_smag},
+            "fator": melhor["fator"]}
+
+
+# ------------------------------------------------------------------ judge
+
+def avaliar(fech: Fechamento, seed: int = AVALIACAO_SEED, passos_grossos: int = 9000,
+            log=print) -> dict:
+    fino, grosso = montar(seed, 2), montar(seed, 1)
+    out = {}
+    out["fino_fp32_smagorinsky"] = rodar(fino, 2 * passos_grossos, "fp32", amostra=200,
+                                         media_desde=2 * passos_grossos // 3)
+    log(f"[avaliar] fine Cd {out['fino_fp32_smagorinsky']['cd']:.4f}")
+    for nome, prec, f in (("grosso_fp32_smagorinsky", "fp32", None),
+                          ("grosso_fp16_smagorinsky", "fp16", None),
+                          ("grosso_fp16_aprendido", "fp16", fech),
+                          ("grosso_fp32_aprendido", "fp32", fech)):
+        out[nome] = rodar(grosso, passos_grossos, prec, fech=f, amostra=100,
+                          media_desde=passos_grossos // 3)
+        log(f"[avaliar] {nome}: Cd {out[nome]['cd']:.4f}, {out[nome]['mlups']:.0f} MLUPS")
+    cd_f = out["fino_fp32_smagorinsky"]["cd"]
+    st_f = out["fino_fp32_smagorinsky"]["st_comprimento"]["lateral"]
+    err = {k: (v["cd"] - cd_f) / cd_f for k, v in out.items() if k.startswith("grosso")}
+    ganho = 16.0 * out["grosso_fp16_aprendido"]["mlups"] / out["fino_fp32_smagorinsky"]["mlups"]
+    aceito = abs(err["grosso_fp16_aprendido"]) <= abs(err["grosso_fp16_smagorinsky"])
+    out["resumo"] = {
+        "cd_fino": cd_f,
+        "erro_cd": err,
+        "st_fino": st_f,
+        "erro_st_lateral": {k: (v["st_comprimento"]["lateral"] - st_f) / st_f
+                            for k, v in out.items() if k.startswith("grosso")},
+        "mlups_equivalente_fino": 16.0 * out["grosso_fp16_aprendido"]["mlups"],
+        "ganho_vs_fino": ganho,
+        "juiz": "aprendido" if aceito else "smagorinsky (o aprendido ficou mais longe do fino)",
+    }
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--passos-finos", type=int, default=12000)
+    ap.add_argument("--passos-grossos", type=int, default=9000)
+    ap.add_argument("--treino", type=int, default=20000, help="Adam steps")
+    ap.add_argument("--formas", type=int, default=len(TREINO_SEEDS))
+    ap.add_argument("--saida", default=None, help="JSON with the report")
+    ap.add_argument("--modelo", default=None, help="where to save the trained closure (.npz)")
+    ap.add_argument("--calibrar", action="store_true",
+                    help="choose the output scale by coarse-vs-fine Cd on training shapes")
+    ap.add_argument("--perfil", action="store_true",
+                    help="only the coarse float16 closure run (for rocprofv3)")
+    ap.add_argument("--carregar", default=None,
+                    help="judge a saved closure (.npz) instead of generating data and training")
+    a = ap.parse_args(argv)
+    if a.perfil:
+        # the coarse float16 closure run alone (kernel profiles), plus the copy
+        # kernel that calibrates the traffic counters
+        from ._lib import copy_bandwidth
+        copy_bandwidth(1 << 30, 3)
+        fech = Fechamento.carregar(a.carregar or MODELO)
+        r = rodar(montar(AVALIACAO_SEED, 1), a.passos_grossos, "fp16", fech=fech)
+        print(json.dumps({k: r[k] for k in ("cd", "mlups", "fallback_frac")}, default=float))
+        return
+    if a.carregar:
+        fech = Fechamento.carregar(a.carregar)
+        cal = calibrar(fech) if a.calibrar else None
+        if a.modelo:
+            fech.salvar(a.modelo)
+        res = avaliar(fech, passos_grossos=a.passos_grossos)
+        res["calibracao"] = cal
+        if a.saida:
+            with open(a.saida, "w") as fh:
+                json.dump(res, fh, indent=1, default=float)
+        print(json.dumps(res["resumo"], default=float))
+        return
+    t0 = time.perf_counter()
+    X, y, info = gerar_dados(TREINO_SEEDS[:a.formas], a.passos_finos)
+    print(f"[fechamento] {X.shape[0]} samples in {time.perf_counter() - t0:.1f} s; nu* mean "
+          f"{y.mean():.3e}, p99 {np.percentile(y, 99):.3e}", flush=True)
+    fech = Fechamento.novo(X, y)
+    t1 = time.perf_counter()
+    curva = fech.treinar(X, y, passos=a.treino, lote=4096, lr=1e-3)
+    print(f"[fechamento] trained {a.treino} Adam steps in {time.perf_counter() - t1:.1f} s, "
+          f"loss {curva[:100].mean():.4f} -> {curva[-100:].mean():.4f}", flush=True)
+    nu_pred, dentro = fech.inferir(X)
+    cal = calibrar(fech) if a.calibrar else None
+    if a.modelo:
+        fech.salvar(a.modelo)
+    r2 = 1.0 - np.mean((nu_pred - y) ** 2) / np.var(y)
+    res = avaliar(fech, passos_grossos=a.passos_grossos)
+    res["calibracao"] = cal
+    res["treino"] = {"amostras": int(X.shape[0]), "formas": info, "r2_treino": float(r2),
+                     "dentro": float(dentro.mean()),
+                     "perda_inicio": float(curva[:100].mean()),
+                     "perda_fim": float(curva[-100:].mean()),
+                     "nu_star_medio": float(y.mean())}
+    if a.saida:
+        with open(a.saida, "w") as fh:
+            json.dump(res, fh, indent=1, default=float)
+    print(json.dumps(res["resumo"], default=float))
+
+
+if __name__ == "__main__":
+    main()
