@@ -77,7 +77,9 @@ __device__ __forceinline__ float mlp_wave(const v8bf& xb, const MlpPacked& W) {
   const v8bf zero = {};
   const v4f zero4 = {0.f, 0.f, 0.f, 0.f};
   float mine = 0.0f;
-#pragma unroll
+  // not unrolled: the weights are re-read from LDS per 16-cell block instead
+  // of being held in registers across all four (one more wave per SIMD)
+#pragma unroll 1
   for (int gs = 0; gs < 4; ++gs) {
     const bool sel = grp == gs;
     // layer 1, cells 16 gs .. 16 gs + 15: A = W1 rows in lane group gs only
@@ -152,10 +154,14 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   }
 
   const Moments M = moments(gi);
-  float e[Q];
-  equilibrium(M, e);
   float pab[6];
-  noneq_stress(gi, e, pab);
+  {
+    // the equilibrium is recomputed after the MLP rather than held across it
+    // (19 registers less at the matrix instructions: one more wave per SIMD)
+    float e[Q];
+    equilibrium(M, e);
+    noneq_stress(gi, e, pab);
+  }
   const float qs = stress_norm(pab);
   const float tau_s = smagorinsky_tau(qs, M.rho, p);
   const bool fluid = live && !wall && ct != ESPELHO_Y && ct != ESPELHO_Z && ct != ENTRADA &&
@@ -163,9 +169,10 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
 
   // features, normalized; outside the trained box the cell keeps tau_s
   float x[kMlpIn];
+  const float inv_rho = 1.0f / M.rho;
 #pragma unroll
-  for (int k = 0; k < 6; ++k) x[k] = pab[k] / M.rho;
-  x[6] = 1.5f * qs / (M.rho * tau_s);
+  for (int k = 0; k < 6; ++k) x[k] = pab[k] * inv_rho;
+  x[6] = 1.5f * qs * inv_rho / tau_s;
   x[7] = (float)wall8[cell] * 0.125f;
   bool inside = true;
   v8bf xb;
@@ -185,6 +192,8 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
 
   float d[Q];
   if (!boundary_cell(ct, gi, M, c.z, p, d)) {
+    float e[Q];
+    equilibrium(M, e);
     trt(gi, e, tau, p, d);
     sponge(c.x, p, d);
   }

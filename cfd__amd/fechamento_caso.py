@@ -212,7 +212,7 @@ def filtrar(macros_f: np.ndarray, fino: dict, grosso: dict, rng, por_instantaneo
     tau_s = 0.5 * (tau0 + np.sqrt(tau0 * tau0 + 18.0 * CS * CS * qs / rhos))
     X = np.empty((sel.size, 8), dtype=np.float32)
     X[:, :6] = pi_rho
-    X[:, 6] = 1.5 * qs / (rhos * tau_s)
+    X[:, 6] = 1.5 * qs / rhos / tau_s
     X[:, 7] = np.asarray(grosso["parede"]).reshape(-1)[sel]
     return X, nus.astype(np.float32)
 
