@@ -197,9 +197,12 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
     trt(gi, e, tau, p, d);
     sponge(c.x, p, d);
   }
-  if (live && !wall) {
+  // wall lanes of a wave that also holds other cells store zeros (never
+  // read), so that every line of dst is written whole (no partial-line fetch)
+  const bool mixed = __ballot(live && !wall) != 0;
+  if (live && (!wall || mixed)) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) S::st(pop_at<LB>(dst, i, cell, g), d[i]);
+    for (int i = 0; i < Q; ++i) S::st(pop_at<LB>(dst, i, cell, g), wall ? 0.0f : d[i]);
   }
   if (MACROS && live)
     macros[cell] = wall ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(M.ux, M.uy, M.uz, M.delta);

@@ -96,9 +96,18 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
   const uint32_t ct = tb & TYPE_MASK;
 
   // Walls are not simulated: nothing ever reads their populations
-  // (wgsl.js:201-204); the macro field is zero there.
-  if (is_wall(ct)) {
+  // (wgsl.js:201-204); the macro field is zero there.  A wave of 64
+  // consecutive cells that mixes walls and other cells still stores zeros in
+  // its wall lanes (never read): every line of dst is then written whole, and
+  // the L2 does not fetch lines from HBM to merge partial writes.
+  const bool wall = is_wall(ct);
+  const bool mixed = __ballot(!wall) != 0;
+  if (wall) {
     if (MACROS) macros[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (mixed) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) S::st(pop_at<LB>(dst, i, cell, g), 0.0f);
+    }
     return;
   }
   if (tb & NEAR_WALL) {
