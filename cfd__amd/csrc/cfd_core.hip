@@ -49,7 +49,7 @@ struct cfd_solver {
   Store store = Store::F32;
   size_t pop_bytes = 0;  // bytes of one population set
   Geom geom{};
-  Tuning tuning{512, 0, kLayoutBlocked, 0};
+  Tuning tuning{256, 8, kLayoutBlocked, 0};  // measured best on the 6.55 M-cell vehicle
   StepParams params{};
   bool configured = false;
   int front = 0;  // 0 = A, 1 = B
@@ -334,6 +334,9 @@ int cfd_set_tuning(cfd_solver* s, const cfd_tuning* t) {
   if (t->layout != kLayoutSoA && t->layout != kLayoutBlocked)
     return fail(CFD_E_ARG, "cfd_set_tuning: layout %d must be 0 (SoA) or 1 (blocked)",
                 t->layout);
+  if (t->remap < 0 || t->remap > 1024 || (t->remap & (t->remap - 1)) != 0)
+    return fail(CFD_E_ARG, "cfd_set_tuning: remap %d must be 0 or a power of two <= 1024",
+                t->remap);
   if (t->slice_pad < 0 || t->slice_pad > (1 << 20))
     return fail(CFD_E_ARG, "cfd_set_tuning: slice_pad %d outside [0, 2^20]", t->slice_pad);
   const size_t eb = elem_bytes(s);
@@ -372,7 +375,7 @@ int cfd_set_tuning(cfd_solver* s, const cfd_tuning* t) {
     s->pop_bytes = nbytes;
   }
   s->tuning.block = t->block;
-  s->tuning.remap = t->remap ? 1 : 0;
+  s->tuning.remap = t->remap > 1 ? t->remap : 0;
   s->tuning.layout = t->layout;
   s->tuning.slice_pad = t->slice_pad;
   return CFD_OK;

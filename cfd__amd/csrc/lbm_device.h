@@ -102,9 +102,15 @@ __device__ __forceinline__ uint32_t logical_block(const Geom& g) {
   const uint32_t b = blockIdx.x;
   return (b % kXcds) * g.per_xcd + b / kXcds;
 }
-__device__ __forceinline__ uint32_t logical_block(uint32_t per_xcd, int remap) {
+// Chunked XCD mapping: XCD k steps through chunks k, k + 8, k + 16, ... of
+// 2^chunk_log2 consecutive workgroups (chunk_log2 = 0: the launch order).  A
+// workgroup's x-neighbour lines at its block boundary then belong to a block
+// stepped by the same XCD, whose L2 already holds them.
+__device__ __forceinline__ uint32_t logical_block_chunked(uint32_t chunk_log2) {
   const uint32_t b = blockIdx.x;
-  return remap ? (b % kXcds) * per_xcd + b / kXcds : b;
+  const uint32_t xcd = b % kXcds, idx = b / kXcds;
+  const uint32_t chunk = idx >> chunk_log2, within = idx & ((1u << chunk_log2) - 1u);
+  return ((chunk * kXcds + xcd) << chunk_log2) + within;
 }
 
 struct Coord {

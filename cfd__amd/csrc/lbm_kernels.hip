@@ -154,8 +154,8 @@ __global__ __launch_bounds__(BLOCK) void k_step(const typename S::T* __restrict_
                                                 typename S::T* __restrict__ dst,
                                                 const uint8_t* __restrict__ tipo,
                                                 float4* __restrict__ macros, Geom g,
-                                                StepParams p, uint32_t per_xcd, int remap) {
-  const uint32_t cell = logical_block(per_xcd, remap) * BLOCK + threadIdx.x;
+                                                StepParams p, uint32_t chunk_log2) {
+  const uint32_t cell = logical_block_chunked(chunk_log2) * BLOCK + threadIdx.x;
   if (cell >= g.N) return;
   step_cell<S, MACROS, LB, false>(cell, gptr(src), gptr(dst), gptr(tipo), macros, g, p,
                                   nullptr);
@@ -514,14 +514,16 @@ static void step_dispatch(const void* src, void* dst, const uint8_t* tipo8, floa
   auto s = static_cast<const typename S::T*>(src);
   auto d = static_cast<typename S::T*>(dst);
   const uint32_t nb = (g.N + tu.block - 1) / tu.block;
-  const uint32_t per = tu.remap ? (nb + kXcds - 1) / kXcds : nb;
-  const dim3 grid(tu.remap ? per * kXcds : nb);
+  uint32_t lg = 0;
+  while ((1 << lg) < tu.remap) ++lg;        // remap = chunk size, a power of two
+  const uint32_t span = kXcds << lg;
+  const dim3 grid(lg ? (nb + span - 1) / span * span : nb);
   if (tu.block == 512)
     hipLaunchKernelGGL((k_step<S, M, 512, L>), grid, dim3(512), 0, st, s, d, tipo8, macros, g,
-                       p, per, tu.remap);
+                       p, lg);
   else
     hipLaunchKernelGGL((k_step<S, M, 256, L>), grid, dim3(256), 0, st, s, d, tipo8, macros, g,
-                       p, per, tu.remap);
+                       p, lg);
 }
 
 hipError_t launch_step(Store sto, const void* src, void* dst, const uint8_t* tipo8,

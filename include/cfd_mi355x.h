@@ -65,7 +65,8 @@ typedef struct cfd_params {
  * the WGSL path has one fixed 64x1x1 workgroup (wgsl.js:186). */
 typedef struct cfd_tuning {
   int32_t block;        /* threads per workgroup: 256 or 512                   */
-  int32_t remap;        /* 1: each XCD streams one contiguous slab of cells    */
+  int32_t remap;        /* 0/1: launch order; K (power of two): each XCD steps
+                           chunks of K consecutive workgroups                   */
   int32_t layout;       /* 0: SoA pop[i][cell]; 1: 512-cell blocks
                            pop[cell/512][i][cell%512] (default; needs a
                            population set below 4 GiB, else SoA is used)        */

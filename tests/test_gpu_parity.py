@@ -208,3 +208,35 @@ def test_rest_offset_closed_form():
         assert abs(off[2]) > 0.1  # the floor contact leaves a downward offset
     finally:
         s.destruir()
+
+
+def test_layouts_and_launch_shapes_are_bit_identical():
+    """The population layout (SoA or 512-cell blocks) and the launch shape
+    (workgroup size, XCD chunking) change addressing only: the populations
+    after 25 steps agree bit for bit."""
+    grid = (72, 40, 36)                 # 103 680 cells: not a multiple of 512
+    tipo, esponja = _tunnel_types(grid, esteira=True)
+    g0 = _random_state(grid, 11)
+    cfg = dict(omegaPlus=1.85, lesCs=0.12, inletU=[0.06, 0.0, 0.0], beltU=[0.06, 0.0, 0.0],
+               esponja=esponja, espessuraCL=5.0)
+    out = {}
+    for t in ({"block": 256, "remap": 8, "layout": 1}, {"block": 512, "remap": 0, "layout": 1},
+              {"block": 256, "remap": 0, "layout": 1}, {"block": 512, "remap": 64, "layout": 1},
+              {"block": 256, "remap": 2, "layout": 0}, {"block": 512, "remap": 0, "layout": 0}):
+        s = _solver().criar({"nx": grid[0], "ny": grid[1], "nz": grid[2]})
+        try:
+            s.ajuste = t
+            s.definirTipos(tipo)
+            s.configurar(**cfg)
+            s.frente = "A"
+            s.definirPopulacoes(g0)
+            s.passo(25)
+            s.atualizarMacros()
+            out[str(t)] = (s.lerPopulacoes(), s.lerMacros())
+        finally:
+            s.destruir()
+    ref_p, ref_m = next(iter(out.values()))
+    wall = np.isin(tipo, [1, 4, 7])
+    for k, (p, m) in out.items():
+        assert np.array_equal(p[:, ~wall], ref_p[:, ~wall]), k
+        assert np.array_equal(m, ref_m), k
