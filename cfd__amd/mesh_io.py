@@ -1,9 +1,12 @@
 """Mesh ingestion for the voxelizer (src/geom/parse.js): STL (binary, found
-by arithmetic, and ASCII), OBJ and PLY.  Returns float32 positions and
-uint32 triangle indices, plus the list of tolerated defects (`avisos`).
+by arithmetic, and ASCII), OBJ, PLY, glTF 2.0 (.gltf with data URIs) and GLB.
+Returns float32 positions and uint32 triangle indices, plus the list of
+tolerated defects (`avisos`); `ler` picks the format from the content.
 """
 from __future__ import annotations
 
+import base64
+import json
 import re
 import struct
 
@@ -224,3 +227,208 @@ def extentos(positions) -> dict:
         lo, hi = np.full(3, np.inf), np.full(3, -np.inf)
     return {"min": lo.tolist(), "max": hi.tolist(), "tamanho": (hi - lo).tolist(),
             "centro": ((lo + hi) / 2).tolist(), "naoFinitos": int((~ok).sum())}
+
+
+# ---------------------------------------------------------------- glTF / GLB
+
+_COMPONENTE = {5120: np.int8, 5121: np.uint8, 5122: np.int16, 5123: np.uint16,
+               5125: np.uint32, 5126: np.float32}
+_N_COMP = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4, "MAT4": 16}
+
+
+def _mat4_mul(a, b):
+    """Column-major 4x4 product a*b, float64 in the reference's summation
+    order (parse.js:348-357)."""
+    o = [0.0] * 16
+    for c in range(4):
+        for r in range(4):
+            o[c * 4 + r] = (a[r] * b[c * 4] + a[4 + r] * b[c * 4 + 1] +
+                            a[8 + r] * b[c * 4 + 2] + a[12 + r] * b[c * 4 + 3])
+    return o
+
+
+def _trs(t, r, s):
+    """Translation, unit quaternion (x, y, z, w), scale -> column-major
+    matrix (parse.js:359-371)."""
+    x, y, z, w = (float(v) for v in r)
+    x2, y2, z2 = x + x, y + y, z + z
+    xx, xy, xz = x * x2, x * y2, x * z2
+    yy, yz, zz = y * y2, y * z2, z * z2
+    wx, wy, wz = w * x2, w * y2, w * z2
+    s = [float(v) for v in s]
+    return [(1 - (yy + zz)) * s[0], (xy + wz) * s[0], (xz - wy) * s[0], 0.0,
+            (xy - wz) * s[1], (1 - (xx + zz)) * s[1], (yz + wx) * s[1], 0.0,
+            (xz + wy) * s[2], (yz - wx) * s[2], (1 - (xx + yy)) * s[2], 0.0,
+            float(t[0]), float(t[1]), float(t[2]), 1.0]
+
+
+def _montar_gltf(doc: dict, buffers: list, avisos: list) -> dict:
+    """Walk the scene, apply node transforms and concatenate every triangle
+    primitive into one mesh (parse.js:388-498)."""
+    pos_acc, idx_acc = [], []
+    total_v = 0
+
+    def dados(i_acc):
+        acc = doc["accessors"][i_acc]
+        dt = _COMPONENTE.get(acc.get("componentType"))
+        n = _N_COMP.get(acc.get("type"))
+        if dt is None or n is None:
+            raise ValueError(f"accessor {i_acc}: unsupported type")
+        count = int(acc["count"])
+        if acc.get("bufferView") is None:
+            return np.zeros(count * n, dtype=dt)     # empty sparse accessor
+        bv = doc["bufferViews"][acc["bufferView"]]
+        buf = buffers[bv.get("buffer", 0)] if bv.get("buffer", 0) < len(buffers) else None
+        if buf is None:
+            raise ValueError("bufferView points to a missing buffer")
+        base = bv.get("byteOffset", 0) + acc.get("byteOffset", 0)
+        tam = np.dtype(dt).itemsize
+        passo = bv.get("byteStride", 0) or 0
+        if not passo or passo == tam * n:
+            return np.frombuffer(buf, dtype=np.dtype(dt).newbyteorder("<"), count=count * n,
+                                 offset=base).astype(dt)
+        # interleaved: de-interleave
+        out = np.empty(count * n, dtype=dt)
+        mv = memoryview(buf)
+        for i in range(count):
+            for k in range(n):
+                o = base + i * passo + k * tam
+                out[i * n + k] = np.frombuffer(mv[o:o + tam], dtype=np.dtype(dt).newbyteorder("<"))[0]
+        return out
+
+    def visitar(i_node, pai):
+        nodes = doc.get("nodes") or []
+        if i_node >= len(nodes) or nodes[i_node] is None:
+            return
+        node = nodes[i_node]
+        local = node["matrix"] if "matrix" in node else _trs(
+            node.get("translation", [0, 0, 0]), node.get("rotation", [0, 0, 0, 1]),
+            node.get("scale", [1, 1, 1]))
+        M = _mat4_mul(pai, [float(v) for v in local])
+        nonlocal total_v
+        if node.get("mesh") is not None:
+            for prim in doc["meshes"][node["mesh"]].get("primitives", []):
+                if prim.get("mode") is not None and prim.get("mode") != 4:
+                    avisos.append(f"primitive with mode={prim['mode']} (not TRIANGLES) skipped")
+                    continue
+                if (prim.get("extensions") or {}).get("KHR_draco_mesh_compression") is not None:
+                    avisos.append("Draco-compressed primitive skipped — export without Draco")
+                    continue
+                i_pos = (prim.get("attributes") or {}).get("POSITION")
+                if i_pos is None:
+                    continue
+                p = dados(i_pos).astype(np.float64).reshape(-1, 3)
+                x, y, z = p[:, 0], p[:, 1], p[:, 2]
+                t = np.empty_like(p)
+                t[:, 0] = M[0] * x + M[4] * y + M[8] * z + M[12]
+                t[:, 1] = M[1] * x + M[5] * y + M[9] * z + M[13]
+                t[:, 2] = M[2] * x + M[6] * y + M[10] * z + M[14]
+                pos_acc.append(t.astype(np.float32).reshape(-1))
+                nv = p.shape[0]
+                if prim.get("indices") is not None:
+                    src = dados(prim["indices"])
+                    ind = (src.astype(np.uint64) + total_v).astype(np.uint32)
+                else:
+                    ind = np.arange(total_v, total_v + nv, dtype=np.uint32)
+                idx_acc.append(ind)
+                total_v += nv
+        for filho in node.get("children", []) or []:
+            visitar(filho, M)
+
+    cenas = doc.get("scenes")
+    cena = cenas[doc.get("scene", 0)] if cenas else None
+    raizes = (cena or {}).get("nodes")
+    if raizes is None:
+        raizes = list(range(len(doc.get("nodes") or [])))
+    ident = [1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0]
+    for r in raizes:
+        visitar(r, ident)
+    if not pos_acc:
+        raise ValueError("glTF without triangle geometry")
+    return {"positions": np.concatenate(pos_acc).astype(np.float32),
+            "indices": np.concatenate(idx_acc).astype(np.uint32)}
+
+
+def _data_uri(uri: str) -> bytes:
+    return base64.b64decode(uri.split(",", 1)[1])
+
+
+def parseGLB(buf: bytes, avisos: list | None = None) -> dict:
+    """Binary glTF (parse.js:500-532): a JSON chunk and BIN chunks."""
+    avisos = [] if avisos is None else avisos
+    if len(buf) < 12 or struct.unpack_from("<I", buf, 0)[0] != 0x46546C67:
+        raise ValueError('no "glTF" magic')
+    versao = struct.unpack_from("<I", buf, 4)[0]
+    if versao != 2:
+        avisos.append(f"GLB version {versao}; the reader is for glTF 2.0")
+    doc, bins = None, []
+    p = 12
+    while p + 8 <= len(buf):
+        tam, tipo = struct.unpack_from("<II", buf, p)
+        dados = buf[p + 8:p + 8 + tam]
+        if tipo == 0x4E4F534A:
+            doc = json.loads(dados.decode("utf-8"))
+        elif tipo == 0x004E4942:
+            bins.append(dados)
+        p += 8 + tam + ((4 - (tam % 4)) % 4)
+    if doc is None:
+        raise ValueError("GLB without a JSON chunk")
+    buffers = []
+    for i, b in enumerate(doc.get("buffers") or []):
+        if b.get("uri") is None:
+            buffers.append(bins[0] if bins else (bins[i] if i < len(bins) else None))
+        elif b["uri"].startswith("data:"):
+            buffers.append(_data_uri(b["uri"]))
+        else:
+            avisos.append(f'external buffer "{b["uri"]}" could not be loaded')
+            buffers.append(None)
+    if not buffers:
+        buffers.append(bins[0] if bins else None)
+    cenas = doc.get("scenes")
+    nome = ((cenas[doc.get("scene", 0)] if cenas else {}) or {}).get("name") or \
+        (doc.get("asset") or {}).get("generator", "")
+    return {**_montar_gltf(doc, buffers, avisos), "formato": "glb", "nome": nome,
+            "avisos": avisos}
+
+
+def parseGLTF(texto: str, avisos: list | None = None) -> dict:
+    """JSON glTF with embedded data URIs (parse.js:534-545)."""
+    avisos = [] if avisos is None else avisos
+    doc = json.loads(texto)
+    buffers = []
+    for b in doc.get("buffers") or []:
+        if (b.get("uri") or "").startswith("data:"):
+            buffers.append(_data_uri(b["uri"]))
+        else:
+            avisos.append(f'external buffer "{b.get("uri")}" could not be loaded — use .glb, '
+                          "which embeds the data")
+            buffers.append(None)
+    cenas = doc.get("scenes")
+    nome = ((cenas[doc.get("scene", 0)] if cenas else {}) or {}).get("name", "")
+    return {**_montar_gltf(doc, buffers, avisos), "formato": "gltf", "nome": nome,
+            "avisos": avisos}
+
+
+def ler(buf: bytes, nomeArquivo: str = "") -> dict:
+    """Any supported format, chosen by content with the file name as the
+    tie-breaker (parse.js:555-586)."""
+    avisos: list = []
+    m = re.search(r"\.([a-z0-9]+)$", nomeArquivo, re.I)
+    ext = m.group(1).lower() if m else ""
+    magia = struct.unpack_from("<I", buf, 0)[0] if len(buf) >= 4 else 0
+    inicio = buf[:1024].decode("latin-1")
+    if magia == 0x46546C67:
+        r = parseGLB(buf, avisos)
+    elif re.match(r"^ply\s", inicio):
+        r = parsePLY(buf, avisos)
+    elif ext == "gltf" or re.match(r'^\s*\{[\s\S]*"asset"', inicio):
+        r = parseGLTF(buf.decode("utf-8"), avisos)
+    elif ext == "obj" or re.search(r"^\s*(v\s|vn\s|vt\s|#|mtllib|o\s|g\s)", inicio, re.M):
+        r = parseSTL(buf, avisos) if _stl_binary_count(buf) else \
+            parseOBJ(buf.decode("utf-8", errors="replace"), avisos)
+    else:
+        r = parseSTL(buf, avisos)
+    n_tri = len(r["indices"]) // 3
+    if not n_tri:
+        raise ValueError(f"{r['formato']}: no triangles")
+    return {**r, "avisos": avisos, "nTriangulos": n_tri, "nVertices": len(r["positions"]) // 3}

@@ -71,6 +71,20 @@ class SolverMI355X:
         if t.size != self.N:
             raise ValueError(f"type field with {t.size}, expected {self.N}")
         check(lib.cfd_set_types(self._h, ptr(t), t.size))
+        self.tipo = t.copy()
+
+    def salvarCampo(self, caminho, uIn: float, formato: str = "vtk") -> None:
+        """The current macro field (lerMacros) to disk: VTK with velocity,
+        delta, Cp and cell type, or raw float32 [N][4] + JSON."""
+        from . import campos
+        m = self.lerMacros()
+        g = {"nx": self.nx, "ny": self.ny, "nz": self.nz}
+        if formato == "vtk":
+            campos.salvar_vtk(caminho, m, g, uIn, getattr(self, "tipo", None))
+        elif formato == "raw":
+            campos.salvar_raw(caminho, m, g, passos=self.passos, uIn=uIn)
+        else:
+            raise ValueError(f"unknown format {formato!r} (vtk or raw)")
 
     # ------------------------------------------------------------ ramp
     def rampa(self, passos: int, aplicarVelocidade: Callable[[float], None]) -> None:

@@ -175,3 +175,28 @@ def test_validation_cases_against_correlations(caso):
     assert abs(r["cd"] - published[0]) <= 0.0005 + 1e-9, r["cd"]
     if published[1] is not None:
         assert abs(r["st"] - published[1]) <= 0.00005 + 1e-9, r["st"]
+
+
+def test_field_output_matches_the_read_back(tmp_path):
+    """salvarCampo writes the macro field lerMacros returns (VTK and raw)."""
+    from cfd__amd import campos
+    g = {"nx": 32, "ny": 16, "nz": 16}
+    corpo = tunel.corpoAnalitico(g, {"tipo": "esfera", "cx": 10, "cy": 8, "cz": 8, "raio": 3})
+    tipo, esponja, _ = tunel.montarTipos(g, corpo, esteira=False)
+    s = _S().criar(g)
+    try:
+        s.definirTipos(tipo)
+        s.configurar(omegaPlus=1.5, lesCs=0.1, inletU=[0.05, 0, 0], esponja=esponja)
+        s.inicializar()
+        s.passo(50)
+        m = s.lerMacros()
+        s.salvarCampo(tmp_path / "f.vtk", 0.05)
+        s.salvarCampo(tmp_path / "f.raw", 0.05, formato="raw")
+    finally:
+        s.destruir()
+    v = campos.ler_vtk(tmp_path / "f.vtk")
+    r, meta = campos.ler_raw(tmp_path / "f.raw")
+    mm = m.reshape(-1, 4)
+    assert np.array_equal(v["velocity"], mm[:, :3]) and np.array_equal(r, mm)
+    assert np.array_equal(v["tipo"], tipo.astype(np.int32)) and meta["passos"] == 50
+    assert np.all(mm[np.isin(tipo, [1, 4, 7])] == 0)          # walls read zero
