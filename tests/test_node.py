@@ -79,3 +79,59 @@ def test_node_run_matches_python_run(tmp_path):
     assert np.allclose(js["off"], off, rtol=0, atol=0)
     assert np.array_equal(np.array(js["f"]), np.array(f))
     assert np.array_equal(mjs, m)
+
+
+def test_validation_helpers_match_python():
+    """The host helpers of cfd__amd/node/validacao.js (tunnel assembly,
+    coefficients, Cd window, Strouhal count, correlations) give the Python
+    restatement's results exactly."""
+    from cfd__amd import forces, validacao
+    g = {"nx": 30, "ny": 14, "nz": 12}
+    code = """
+const v = require('./cfd__amd/node/validacao.js');
+const g = {nx: 30, ny: 14, nz: 12};
+const corpo = v.corpoAnalitico(g, {tipo: 'esfera', cx: 9, cy: 7, cz: 6, raio: 3.3});
+const cil = v.corpoAnalitico(g, {tipo: 'cilindro', cx: 9, cy: 7, cz: 6, raio: 2.2});
+const m = v.montarTipos(g, corpo, false);
+const j = new v.JanelaCd(5);
+[1.1, 1.3, NaN, 1.2, 1.25, 1.0, 0.9].forEach((x) => j.adicionar(x));
+const serie = Array.from({length: 200}, (_, k) => Math.sin(2 * Math.PI * k / 17.3) + 0.1);
+console.log(JSON.stringify({tipo: Array.from(m.tipo), esponja: m.esponja,
+  cil: Array.from(cil), area: v.areaFrontalCorpo(corpo, g),
+  co: v.coeficientes([0.02, -0.003, 0.004], 0.05, 41, g, 2.5),
+  media: j.media, desvio: j.desvio, st: v.strouhal(serie, 100, 16, 0.05),
+  cd: [v.esferaCd(50), v.cilindroCd(100), v.cilindroStrouhal(100), v.uLbParaOmega(100, 16)]}));
+"""
+    r = json.loads(_node(code).strip().splitlines()[-1])
+    corpo = tunel.corpoAnalitico(g, {"tipo": "esfera", "cx": 9, "cy": 7, "cz": 6, "raio": 3.3})
+    cil = tunel.corpoAnalitico(g, {"tipo": "cilindro", "cx": 9, "cy": 7, "cz": 6, "raio": 2.2})
+    tipo, esp, _ = tunel.montarTipos(g, corpo, esteira=False)
+    assert r["tipo"] == tipo.tolist() and r["esponja"] == esp and r["cil"] == cil.tolist()
+    assert r["area"] == tunel.areaFrontalCorpo(corpo, g)
+    co = forces.coeficientes([0.02, -0.003, 0.004], type("U", (), {"uLb": 0.05})(), 41, g,
+                             corrigirBloqueio=True, theta=2.5)
+    assert r["co"]["cd"] == co["cd"] and r["co"]["cl"] == co["cl"] and r["co"]["cs"] == co["cs"]
+    j = forces.JanelaCd(5)
+    for x in [1.1, 1.3, float("nan"), 1.2, 1.25, 1.0, 0.9]:
+        j.adicionar(x)
+    assert r["media"] == j.media and r["desvio"] == j.desvio
+    import math
+    serie = [math.sin(2 * math.pi * k / 17.3) + 0.1 for k in range(200)]
+    assert r["st"]["st"] == forces.strouhal(serie, 100, 16, 0.05)["st"]
+    assert r["cd"] == [validacao.esferaCd(50), validacao.cilindroCd(100),
+                       validacao.cilindroStrouhal(100), validacao.uLbParaOmega(100, 16)]
+
+
+@pytest.mark.gpu
+def test_node_validation_page_matches_python(tmp_path):
+    """tests/validacao.html's loop as a Node script reproduces the Python
+    headless case (and so the reference's published Cd) to the last digit."""
+    from cfd__amd.caso import rodar_caso
+    from cfd__amd.validacao import CASOS
+    out = subprocess.run([NODE, "cfd__amd/node/validacao.js", "esfera-re50"], cwd=ROOT,
+                         check=True, text=True, stdout=subprocess.PIPE, timeout=300).stdout
+    js = json.loads(out.strip().splitlines()[-1])
+    py = rodar_caso([c for c in CASOS if c["id"] == "esfera-re50"][0])
+    assert js["passos"] == py["passos"] and js["areaFrontal"] == py["areaFrontal"]
+    assert js["cd"] == py["cd"], (js["cd"], py["cd"])
+    assert abs(js["cd"] - 1.480) <= 0.0005
