@@ -45,7 +45,7 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--precision", choices=("fp32", "fp16"), default="fp32")
+    ap.add_argument("--precision", choices=("fp32", "fp16", "fp16sr"), default="fp32")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="budget of the CPU-baseline sample (0 disables it)")
     return ap.parse_args()

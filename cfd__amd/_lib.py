@@ -14,6 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libcfdmi.so"
 
 CFD_STORE_FP32 = 0
 CFD_STORE_FP16 = 1
+CFD_ROUND_STOCHASTIC = 2
 
 
 class CFDError(RuntimeError):

@@ -53,6 +53,7 @@ struct cfd_solver {
   StepParams params{};
   bool configured = false;
   int front = 0;  // 0 = A, 1 = B
+  uint32_t sr_step = 0;  // steps taken (seeds the stochastic float16 rounding)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void* pop[2] = {nullptr, nullptr};
@@ -110,7 +111,7 @@ float dotc_host(int i, const float v[3]) {
   return acc;
 }
 
-size_t elem_bytes(const cfd_solver* s) { return s->store == Store::F16 ? 2 : 4; }
+size_t elem_bytes(const cfd_solver* s) { return s->store == Store::F32 ? 4 : 2; }
 
 }  // namespace
 
@@ -185,7 +186,8 @@ int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
   if (N >= (1ull << 30))
     return fail(CFD_E_ARG, "cfd_create: %llu cells exceed the 2^30 limit of one handle",
                 (unsigned long long)N);
-  if (flags != CFD_STORE_FP32 && flags != CFD_STORE_FP16)
+  if (flags != CFD_STORE_FP32 && flags != CFD_STORE_FP16 &&
+      flags != (CFD_STORE_FP16 | CFD_ROUND_STOCHASTIC))
     return fail(CFD_E_ARG, "cfd_create: unknown storage flags 0x%x", flags);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -196,8 +198,9 @@ int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
   s->ny = ny;
   s->nz = nz;
   s->N = (size_t)N;
-  s->store = flags == CFD_STORE_FP16 ? Store::F16 : Store::F32;
-  const size_t eb = s->store == Store::F16 ? 2 : 4;
+  s->store = flags == CFD_STORE_FP32 ? Store::F32
+                                     : (flags & CFD_ROUND_STOCHASTIC ? Store::F16S : Store::F16);
+  const size_t eb = s->store == Store::F32 ? 4 : 2;
   s->geom = make_geom(nx, ny, nz, s->tuning.layout, (uint32_t)s->tuning.slice_pad);
   if (s->geom.pop_elems * eb >= (1ull << 32)) {
     // the blocked layout addresses a set with 32-bit byte offsets
@@ -409,6 +412,7 @@ int cfd_get_front(cfd_solver* s, int* front) {
 static int enqueue_steps(cfd_solver* s, int n) {
   for (int k = 0; k < n; ++k) {
     const int f = s->front;
+    s->params.sr_seed = ++s->sr_step * 0x9E3779B9u;   // a new rounding draw per step
     if (s->mlp)
       HIPCK(launch_step_closure(s->store, s->pop[f], s->pop[f ^ 1], s->tipo8, s->wall8, s->mlp,
                                 s->macros, k == n - 1, s->geom, s->params, s->fallbacks,

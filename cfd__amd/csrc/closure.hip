@@ -201,8 +201,10 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   // read), so that every line of dst is written whole (no partial-line fetch)
   const bool mixed = __ballot(live && !wall) != 0;
   if (live && (!wall || mixed)) {
+    const uint32_t h = S::kStochastic ? sr_hash(cell, p.sr_seed) : 0u;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) S::st(pop_at<LB>(dst, i, cell, g), wall ? 0.0f : d[i]);
+    for (int i = 0; i < Q; ++i)
+      store_pop<S>(pop_at<LB>(dst, i, cell, g), wall ? 0.0f : d[i], h, i);
   }
   if (MACROS && live)
     macros[cell] = wall ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(M.ux, M.uy, M.uz, M.delta);

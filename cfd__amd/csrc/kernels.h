@@ -45,6 +45,7 @@ struct StepParams {
   float sponge_start, sponge_len_m;  // len_m = max(len, 1.0) (ir.js:461)
   float bl, bl_m;                    // bl_m = max(bl, 1.0)   (wgsl.js:217)
   float tinf[19];   // free-stream equilibrium of the sponge  (ir.js:464-475)
+  uint32_t sr_seed; // per-step seed of the stochastic float16 rounding
 };
 
 Geom make_geom(int nx, int ny, int nz, int layout, uint32_t slice_pad);
@@ -60,8 +61,9 @@ struct Tuning {
   int slice_pad;    // SoA only: extra elements between population slices
 };
 
-// Population storage: float32 (152 B/cell/step) or scaled float16 (76 B).
-enum class Store { F32 = 0, F16 = 1 };
+// Population storage: float32 (152 B/cell/step) or scaled float16 (76 B),
+// the latter rounded to nearest or stochastically by the step.
+enum class Store { F32 = 0, F16 = 1, F16S = 2 };
 
 hipError_t launch_prep_types(const uint32_t* tipo32, uint8_t* tipo8, const Geom& g,
                              hipStream_t st);

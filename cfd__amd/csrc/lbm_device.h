@@ -32,6 +32,7 @@ __device__ __forceinline__ GLOBAL T* gptr(T* p) {
 }
 
 struct StoreF32 {
+  static constexpr bool kStochastic = false;
   using T = float;
   template <class P>
   static __device__ __forceinline__ float ld(P p) { return *p; }
@@ -43,6 +44,7 @@ struct StoreF32 {
 // the scaling itself adds no rounding): |g| up to 16 stays finite and values
 // down to 1.5e-8 stay normal.  Collision still runs in float32.
 struct StoreF16 {
+  static constexpr bool kStochastic = false;
   using T = _Float16;
   template <class P>
   static __device__ __forceinline__ float ld(P p) {
@@ -51,6 +53,48 @@ struct StoreF16 {
   template <class P>
   static __device__ __forceinline__ void st(P p, float v) { *p = static_cast<T>(v * 4096.0f); }
 };
+
+// The same float16 storage with stochastic rounding in the step's stores: the
+// float32 value is rounded up or down to a neighbouring float16 with
+// probability proportional to its distance (13 random bits added below the
+// float16 mantissa, then truncated), so the stored value is unbiased.  A
+// deterministic rounding loses every update smaller than half a float16 ulp
+// (~2e-4 relative): a slowly decaying flow then stops decaying.  Outside the
+// float16 normal range the store rounds to nearest.  Loads, init and
+// conversions are those of StoreF16.
+struct StoreF16S : StoreF16 {
+  static constexpr bool kStochastic = true;
+  template <class P>
+  static __device__ __forceinline__ void st_sr(P p, float v, uint32_t bits13) {
+    uint32_t u = __float_as_uint(v * 4096.0f);
+    const uint32_t e = (u >> 23) & 0xffu;
+    if (e - 113u < 29u) u = (u + bits13) & 0xffffe000u;   // 2^-14 <= |x| < 2^15
+    *p = static_cast<T>(__uint_as_float(u));
+  }
+};
+
+// 13 random bits for population i of a cell: one avalanche hash per cell and
+// step, then a cheap per-direction mix.
+__device__ __forceinline__ uint32_t sr_hash(uint32_t cell, uint32_t seed) {
+  uint32_t h = cell * 0x9E3779B1u ^ seed;
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
+__device__ __forceinline__ uint32_t sr_bits(uint32_t h, int i) {
+  return ((h ^ ((uint32_t)i * 0x632BE5ABu)) * 0x85EBCA6Bu) >> 19;
+}
+
+// Store population i of cell c: rounded to nearest, or stochastically for
+// StoreF16S (h = sr_hash of the cell for this step).
+template <class S, class P>
+__device__ __forceinline__ void store_pop(P p, float v, uint32_t h, int i) {
+  if constexpr (S::kStochastic) S::st_sr(p, v, sr_bits(h, i));
+  else S::st(p, v);
+}
 
 // ------------------------------------------------------------------ indexing
 
@@ -343,8 +387,12 @@ __device__ __forceinline__ bool boundary_cell(uint32_t ct, const float (&gi)[Q],
       using S = StoreF32;                                                  \
       if ((g).lb) { constexpr int L = kBlockLog2; __VA_ARGS__ }            \
       else { constexpr int L = 0; __VA_ARGS__ }                            \
-    } else {                                                               \
+    } else if ((sto) == Store::F16) {                                      \
       using S = StoreF16;                                                  \
+      if ((g).lb) { constexpr int L = kBlockLog2; __VA_ARGS__ }            \
+      else { constexpr int L = 0; __VA_ARGS__ }                            \
+    } else {                                                               \
+      using S = StoreF16S;                                                 \
       if ((g).lb) { constexpr int L = kBlockLog2; __VA_ARGS__ }            \
       else { constexpr int L = 0; __VA_ARGS__ }                            \
     }                                                                      \

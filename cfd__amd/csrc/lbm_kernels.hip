@@ -144,8 +144,9 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
     sponge(c.x, p, d);
   }
 
+  const uint32_t h = S::kStochastic ? sr_hash(cell, p.sr_seed) : 0u;
 #pragma unroll
-  for (int i = 0; i < Q; ++i) S::st(pop_at<LB>(dst, i, cell, g), d[i]);
+  for (int i = 0; i < Q; ++i) store_pop<S>(pop_at<LB>(dst, i, cell, g), d[i], h, i);
   if (MACROS) macros[cell] = make_float4(M.ux, M.uy, M.uz, M.delta);
 }
 

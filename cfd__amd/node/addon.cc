@@ -135,11 +135,20 @@ void finalize(napi_env, void* data, void*) {
 napi_value Create(napi_env env, napi_callback_info info) {
   napi_value a[4];
   if (!args(env, info, 4, a)) return nullptr;
-  bool fp16 = false;
-  napi_get_value_bool(env, a[3], &fp16);
+  // a boolean (float16 or not) or the cfd_create flags as a number
+  uint32_t flags = CFD_STORE_FP32;
+  napi_valuetype t;
+  NAPI_OK(napi_typeof(env, a[3], &t));
+  if (t == napi_boolean) {
+    bool fp16 = false;
+    napi_get_value_bool(env, a[3], &fp16);
+    flags = fp16 ? CFD_STORE_FP16 : CFD_STORE_FP32;
+  } else {
+    flags = (uint32_t)num(env, a[3]);
+  }
   cfd_solver* s = nullptr;
   CFD_OK_OR_THROW(cfd_create((int)num(env, a[0]), (int)num(env, a[1]), (int)num(env, a[2]),
-                             fp16 ? CFD_STORE_FP16 : CFD_STORE_FP32, &s));
+                             flags, &s));
   Handle* h = new Handle{s};
   napi_value ext;
   NAPI_OK(napi_create_external(env, h, finalize, nullptr, &ext));

@@ -36,8 +36,9 @@ class SolverMI355X {
   /** Allocate a solver on the GPU (solver.js:93-124).  precisao 'fp32' | 'fp16'. */
   static async criar(ctx, grade, precisao) {
     const p = precisao || 'fp32';
-    if (p !== 'fp32' && p !== 'fp16') throw new Error(`precisao deve ser fp32 ou fp16: ${p}`);
-    const h = addon.create(grade.nx, grade.ny, grade.nz, p === 'fp16');
+    const flags = { fp32: 0, fp16: 1, fp16sr: 3 }[p];
+    if (flags === undefined) throw new Error(`precisao deve ser fp32, fp16 ou fp16sr: ${p}`);
+    const h = addon.create(grade.nx, grade.ny, grade.nz, flags);
     return new SolverMI355X(h, grade, p);
   }
 

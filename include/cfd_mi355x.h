@@ -34,6 +34,8 @@ extern "C" {
 /* Storage flags for cfd_create. */
 #define CFD_STORE_FP32 0u   /* populations g_i = f_i - w_i stored as float32    */
 #define CFD_STORE_FP16 1u   /* populations stored as scaled float16 deviations */
+#define CFD_ROUND_STOCHASTIC 2u /* with FP16: the step rounds each stored value up or
+                                   down at random in proportion (unbiased)      */
 
 /* Cell types, identical to CELULA in src/core/emit/ir.js:71-80. */
 #define CFD_FLUIDO 0u
@@ -89,7 +91,8 @@ typedef struct cfd_vox_stats {
 /* SolverWebGPU.criar (solver.js:93-124) + _alocar (132-174).  Allocates two
  * population sets (A and B), the cell-type field and the macro field, all
  * zero-filled: g_i = 0 is fluid at rest with rho = 1 exactly (lattice.js:32-34).
- * The front starts at A.  flags: CFD_STORE_FP32 or CFD_STORE_FP16. */
+ * The front starts at A.  flags: CFD_STORE_FP32, CFD_STORE_FP16 or
+ * CFD_STORE_FP16 | CFD_ROUND_STOCHASTIC. */
 int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver **out);
 
 /* SolverWebGPU.configurar (solver.js:277-293). */
