@@ -32,7 +32,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0          # MI355X published HBM3E bandwidth
-BYTES_PER_CELL = {"fp32": 152, "fp16": 76}   # 19 pops read + 19 written
+BYTES_PER_CELL = {"fp32": 152, "fp16": 76, "fp16sr": 76}   # 19 pops read + 19 written
 GRID = (320, 160, 128)
 U_IN = 0.08
 RE_REQUESTED = 1.0e5
