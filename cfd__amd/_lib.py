@@ -46,7 +46,7 @@ class TunnelSpec(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("block", C.c_int32), ("tile_cells", C.c_int32), ("wg_per_cu", C.c_int32),
-                ("max_active_tiles", C.c_int32)]
+                ("max_active_tiles", C.c_int32), ("tiles_per_grab", C.c_int32)]
 
 
 class Mlp(C.Structure):

@@ -62,7 +62,8 @@ class LoteMI355X:
         if ajuste is not None:
             t = BatchTuning(int(ajuste["block"]), int(ajuste["tile_cells"]),
                             int(ajuste.get("wg_per_cu", 0)),
-                            int(ajuste.get("max_active_tiles", 0)))
+                            int(ajuste.get("max_active_tiles", 0)),
+                            int(ajuste.get("tiles_per_grab", 0)))
         ms = C.c_double()
         check(lib.cfd_batch_run(self._h, int(passos), int(amostra),
                                 C.byref(t) if t is not None else None, C.byref(ms)))
@@ -103,6 +104,7 @@ class LoteMI355X:
                 for tile in (512, 1024, 2048, 4096):
                     for mult in (8, 32):
                         candidatos.append({"block": block, "tile_cells": tile, "wg_per_cu": per,
+                                           "tiles_per_grab": 1,
                                            "max_active_tiles": per * cus * mult * 4096 // tile})
         medidas = []
         for c in candidatos:

@@ -62,7 +62,7 @@ struct cfd_batch {
   double* samples = nullptr;
   unsigned long long* clocks = nullptr;
   size_t clocks_len = 0;
-  cfd_batch_tuning tuning{512, 1024, 0, 0};
+  cfd_batch_tuning tuning{512, 512, 0, 0, 1};
 };
 
 namespace {
@@ -216,6 +216,9 @@ int cfd_batch_run(cfd_batch* b, int steps, int sample_every, const cfd_batch_tun
   cfd_batch_tuning t = tu ? *tu : b->tuning;
   if (t.block != 256 && t.block != 512)
     return bfail(CFD_E_ARG, "cfd_batch_run: block %d must be 256 or 512", t.block);
+  if (t.tiles_per_grab < 0 || t.tiles_per_grab > 64)
+    return bfail(CFD_E_ARG, "cfd_batch_run: tiles_per_grab %d outside [0, 64]",
+                 t.tiles_per_grab);
   if (t.tile_cells < 64 || t.tile_cells > (1 << 20))
     return bfail(CFD_E_ARG, "cfd_batch_run: tile_cells %d outside [64, 2^20]", t.tile_cells);
   int per_cu = 0, cus = 0;
@@ -275,6 +278,7 @@ int cfd_batch_run(cfd_batch* b, int steps, int sample_every, const cfd_batch_tun
   L.sample_every = sample_every;
   L.max_samples = b->max_samples;
   L.parity0 = (uint32_t)b->cur;
+  L.grab = t.tiles_per_grab > 0 ? t.tiles_per_grab : 1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);

@@ -116,6 +116,7 @@ struct BatchLaunch {       // everything k_batch_run reads besides the tunnels
   double* samples;             // [tunnel][sample][3]
   unsigned long long* clocks;  // [0] launch start, [1 + r] end of round r
   int nphases, tile_cells, sample_every, max_samples;
+  int grab;                    // consecutive tiles taken per counter increment
   uint32_t parity0;            // every tunnel's current state is in pop[parity0]
 };
 hipError_t batch_max_grid(int block, int* max_blocks_per_cu, int* cus);

@@ -196,7 +196,8 @@ __device__ __forceinline__ void block_sum_into(double (&v)[3], double (&out)[3])
 // tunnel's parity (A or B as source) is its own step count, so tunnels
 // admitted at different rounds coexist.
 //
-// Workgroups take tiles from a per-round counter, in list order, the way the
+// Workgroups take tiles (`grab` consecutive ones at a time) from a per-round
+// counter, in list order, the way the
 // hardware dispatcher hands out workgroups of an ordinary launch: the tiles
 // in flight stay a contiguous window of the list, so the z-neighbour planes
 // a tile pulls from are being read by its neighbours in the window at the
@@ -218,12 +219,15 @@ __global__ __launch_bounds__(BLOCK) void k_batch_run(const BatchTunnel* __restri
     const BatchPhase P = L.phases[ph];
     for (uint32_t r = P.r0; r < P.r1; ++r) {
       uint32_t* ctr = L.counters + r;
-      if (threadIdx.x == 0) s_tile[0] = atomicAdd(ctr, 1u);
+      const uint32_t grab = (uint32_t)L.grab;
+      if (threadIdx.x == 0) s_tile[0] = atomicAdd(ctr, grab);
       __syncthreads();
-      uint32_t t = s_tile[0];
-      for (uint32_t it = 1; t < P.total_tiles; ++it) {
+      uint32_t t0 = s_tile[0];
+      for (uint32_t it = 1; t0 < P.total_tiles; ++it) {
         uint32_t next = 0;
-        if (threadIdx.x == 0) next = atomicAdd(ctr, 1u);  // consumed after the tile
+        if (threadIdx.x == 0) next = atomicAdd(ctr, grab);  // consumed after the tiles
+        const uint32_t tend = min(t0 + grab, P.total_tiles);
+        for (uint32_t t = t0; t < tend; ++t) {
         const BatchEntry E = L.entries[P.first + L.tile_entry[P.table + t]];
         const BatchTunnel& D = tun[E.tunnel];
         const uint32_t step = r - E.start_round;
@@ -256,10 +260,11 @@ __global__ __launch_bounds__(BLOCK) void k_batch_run(const BatchTunnel* __restri
             step_cell<StoreF32, false, kBlockLog2, false>(cell, src, dst, tipo, nullptr,
                                                           D.g, D.p, nullptr);
         }
+        }
         // ping-pong slot: a wave still reading the previous slot is not raced
         if (threadIdx.x == 0) s_tile[it & 1] = next;
         __syncthreads();
-        t = s_tile[it & 1];
+        t0 = s_tile[it & 1];
       }
       grid.sync();
       if (blockIdx.x == 0 && threadIdx.x == 0) L.clocks[1 + r] = wall_clock64();

@@ -225,7 +225,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--saida", default=None)
     ap.add_argument("--ajuste", default=None,
-                    help="block,tile_cells,wg_per_cu,max_active_tiles (skips the autotuner)")
+                    help="block,tile_cells,wg_per_cu,max_active_tiles[,tiles_per_grab] "
+                         "(skips the autotuner)")
     ap.add_argument("--copia", action="store_true",
                     help="also run the 1 GiB copy kernel (counter calibration for profiles)")
     a = ap.parse_args(argv)
@@ -234,8 +235,9 @@ def main(argv=None):
         copy_bandwidth(1 << 30, 3)
     ajuste = None
     if a.ajuste:
-        b, t, w, m = (int(x) for x in a.ajuste.split(","))
-        ajuste = {"block": b, "tile_cells": t, "wg_per_cu": w, "max_active_tiles": m}
+        v = [int(x) for x in a.ajuste.split(",")]
+        ajuste = dict(zip(("block", "tile_cells", "wg_per_cu", "max_active_tiles",
+                           "tiles_per_grab"), v))
     out = rodar_lote(gerar_casos(a.casos, a.seed), a.passos, a.amostra, ajuste=ajuste)
     if a.saida:
         with open(a.saida, "w") as fh:

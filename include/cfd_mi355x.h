@@ -242,6 +242,8 @@ typedef struct cfd_batch_tuning {
   int32_t tile_cells;       /* consecutive cells per tile (64 .. 2^20)       */
   int32_t wg_per_cu;        /* resident workgroups per CU; 0 = maximum       */
   int32_t max_active_tiles; /* admission cap; 0 = 8 tiles per workgroup      */
+  int32_t tiles_per_grab;   /* consecutive tiles a workgroup takes at once;
+                               0 = 1                                          */
 } cfd_batch_tuning;
 
 /* Allocates every tunnel (blocked float32 layout), uploads its types and
