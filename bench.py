@@ -169,7 +169,7 @@ def cpu_baseline(case, budget_s: float):
 def main():
     args = parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note, not built).", file=sys.stderr)
         sys.exit(2)
     from cfd__amd import _lib

@@ -2,7 +2,7 @@
 the cases of voxel_cases.py.
 
 Runs only in the development container (it reads /root/reference through
-run_reference_voxelizer.js under Node); the GPU box never runs it.  The JSON
+run_reference_voxelizer.js under Node); the GPU test machines never run it.  The JSON
 holds data only: per case the mesh digest, the grid and options, sha256 of
 the reference's tipo (uint8) and sdf (float32) arrays, its counts, and for the
 small cases the full tipo as run-length pairs.

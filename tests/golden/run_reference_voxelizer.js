@@ -1,5 +1,5 @@
 // Fixture generator (runs in the development container only, never on the
-// GPU box): evaluates the reference's voxelizar() from
+// GPU test machines): evaluates the reference's voxelizar() from
 // /root/reference/src/geom/voxel.js as an ES module and runs it on the meshes
 // listed in a manifest written by make_voxel_golden.py.  Prints one JSON
 // object per case with hashes of the outputs; nothing of the reference's text
