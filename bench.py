@@ -43,12 +43,21 @@ CHUNK = 100
 def parse_args():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 20000 for the vehicle, 4000 for the sphere)")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--precision", choices=("fp32", "fp16", "fp16sr"), default="fp32")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="budget of the CPU-baseline sample (0 disables it)")
-    return ap.parse_args()
+    ap.add_argument("--workload", choices=("vehicle", "sphere"), default="vehicle",
+                    help="vehicle: 320x160x128 (the headline); sphere: the 160x80x64 "
+                         "icosphere case, 4000 steps")
+    ap.add_argument("--campos", default=None,
+                    help="sphere workload: write the final macro field (VTK) here")
+    args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 4000 if args.workload == "sphere" else 20000
+    return args
 
 
 def traffic_from_profiles(precision: str):
@@ -166,12 +175,117 @@ def cpu_baseline(case, budget_s: float):
                       f"{el:.1f} s"}
 
 
+def sphere_case():
+    """160x80x64, seeded 20 480-triangle icosphere D = 16 at (48, 40, 32),
+    free-slip sides, top and floor (the body in free air), u = 0.05, nu for
+    Re = 100 (omega 1.9084, just above the reference's 1.90 ceiling), Cs 0.1."""
+    from cfd__amd import meshes, tunel
+    from cfd__amd.units import Units
+    from cfd__amd.voxel import voxelizar
+
+    nx, ny, nz = 160, 80, 64
+    g = {"nx": nx, "ny": ny, "nz": nz}
+    pos, idx = meshes.icosphere(5, 16.0, (48.0, 40.0, 32.0), 1)
+    voxelizar(pos, idx, g, banda=1.5, selarFrestas=True, comSdf=False)   # device warm-up
+    t0 = time.perf_counter()
+    vox = voxelizar(pos, idx, g, banda=1.5, selarFrestas=True, comSdf=False)
+    vox_ms = (time.perf_counter() - t0) * 1e3
+    tipo, esponja, cont = tunel.montarTipos(g, vox["tipo"], esteira=False)
+    tipo.reshape(nz, ny, nx)[0, :, 1:nx - 1] = 6
+    nu = 0.05 * 16 / 100
+    units = Units(lengthM=0.016, speedMs=0.1, cellsPerLength=16, uLb=0.05, lesCs=0.1)
+    return dict(grid=g, tipo=tipo, esponja=esponja, area=tunel.areaFrontalCorpo(vox["tipo"], g),
+                omega=1 / (3 * nu + 0.5), u=0.05, units=units, n_tri=len(idx) // 3,
+                voxel_ms=vox_ms)
+
+
+def main_sphere(args):
+    from cfd__amd.forces import JanelaCd, coeficientes
+    from cfd__amd.solver import SolverMI355X
+
+    c = sphere_case()
+    g = c["grid"]
+    cells = g["nx"] * g["ny"] * g["nz"]
+    cfg = dict(omegaPlus=c["omega"], lesCs=0.1, inletU=[c["u"], 0, 0], esponja=c["esponja"])
+    s = SolverMI355X.criar(g, precisao=args.precision)
+    try:
+        s.definirTipos(c["tipo"])
+        s.configurar(**cfg)
+        s.calibrarRepouso()
+        s.inicializar()
+        s.passo(max(args.warmup, 0))
+        s.sincronizar()
+        s.inicializar()          # the timed run starts from the initial state again
+        jan = JanelaCd(10 ** 6)
+        kernel_ms = 0.0
+        t0 = time.perf_counter()
+        done = 0
+        while done < args.steps:
+            n = min(CHUNK, args.steps - done)
+            kernel_ms += s.passoCronometrado(n)
+            done += n
+            if done > args.steps - 1000:
+                co = coeficientes(s.medirForcas(), c["units"], c["area"], g,
+                                  corrigirBloqueio=True, theta=2.5)
+                jan.adicionar(co["cd"])
+        s.sincronizar()
+        wall = time.perf_counter() - t0
+        if args.campos:
+            s.salvarCampo(args.campos, c["u"])
+        mass, bad = s.massa()
+    finally:
+        s.destruir()
+    cpu = None
+    if args.cpu_seconds > 0:
+        import oracle
+        par = oracle.params_array(nx=g["nx"], **cfg)
+        grid = (g["nx"], g["ny"], g["nz"])
+        a = oracle.init(grid, c["tipo"], par)
+        b = a.copy()
+        lib, P = oracle.lib(), oracle._p
+        t = np.ascontiguousarray(c["tipo"], dtype=np.uint32)
+        n = 0
+        t1 = time.perf_counter()
+        while time.perf_counter() - t1 < args.cpu_seconds and n < 4000:
+            lib.orc_step_f32(*grid, P(a), P(b), P(t), P(par), None)
+            a, b = b, a
+            n += 1
+        el = time.perf_counter() - t1
+        cpu = {"value": cells * n / el / 1e6, "unit": "MLUPS", "cores": oracle.threads(),
+               "kind": "port", "sample": f"oracle float32 step, same case, {n} steps, {el:.1f} s"}
+    kernel_avg_s = kernel_ms / 1e3 / args.steps
+    bpc = BYTES_PER_CELL[args.precision]
+    print(json.dumps({
+        "metric": "MLUPS (D3Q19 TRT-LES, 0.82M cells, sphere)", "unit": "MLUPS",
+        "value": round(cells * args.steps / wall / 1e6, 1), "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f16-storage/f32-collide",
+        "data": f"synthetic (seeded {c['n_tri']}-triangle icosphere, GPU-voxelized)",
+        "config": {"workload": f"plain D3Q19 TRT(3/16)+Smagorinsky(0.1) {args.precision} "
+                               "sphere D16 160x80x64, u_in=0.05, Re=100",
+                   "omega_plus": c["omega"], "cells": cells},
+        "roofline": {"bound": "hbm", "achieved": round(bpc * cells / kernel_avg_s / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bpc * cells / kernel_avg_s / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_kernel_us": round(kernel_avg_s * 1e6, 2),
+                     "note": "the 125 MB population set fits the 256 MB last-level cache"},
+        "cpu_baseline": cpu,
+        "results": {"cd_mean_last_1000": jan.media, "cd_std": jan.desvio,
+                    "frontal_area_cells": c["area"], "mass_final": mass, "nonfinite_cells": bad,
+                    "voxelize_ms": round(c["voxel_ms"], 2),
+                    "campos": args.campos},
+    }))
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note, not built).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "sphere":
+        return main_sphere(args)
     from cfd__amd import _lib
 
     precision = args.precision

@@ -369,24 +369,55 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* out) {
 // (the 2 w_i c_i term does not cancel per link: lattice.js:37-45).  Only
 // cells flagged BODY_LINK look at their neighbours; accumulation is float64.
 template <class S, int LB>
-__global__ __launch_bounds__(kBlock) void k_forces(const typename S::T* __restrict__ pop,
-                                                   const uint8_t* __restrict__ tipo, Geom g,
-                                                   double* __restrict__ partials) {
-  double f[3] = {0.0, 0.0, 0.0};
-  for (uint32_t cell = blockIdx.x * kBlock + threadIdx.x; cell < g.N;
-       cell += gridDim.x * kBlock) {
-    if (!(tipo[cell] & BODY_LINK)) continue;
-    const Coord c = split(cell, g);
-    const Wrap w = wrap_offsets(c, g);
+__device__ __forceinline__ void link_force(const GLOBAL typename S::T* __restrict__ pop,
+                                           const GLOBAL uint8_t* __restrict__ tipo,
+                                           const Geom& g, uint32_t cell, double (&f)[3]) {
+  const Coord c = split(cell, g);
+  const Wrap w = wrap_offsets(c, g);
 #pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      const uint32_t nb = (uint32_t)((int)cell + nb_offset(w, cx(i), cy(i), cz(i)));
-      if ((tipo[nb] & TYPE_MASK) != SOLIDO) continue;
-      const double q = 2.0 * ((double)S::ld(pop_at<LB>(pop, i, cell, g)) + wd(i));
-      if (cx(i)) f[0] += cx(i) > 0 ? q : -q;
-      if (cy(i)) f[1] += cy(i) > 0 ? q : -q;
-      if (cz(i)) f[2] += cz(i) > 0 ? q : -q;
+  for (int i = 1; i < Q; ++i) {
+    const uint32_t nb = (uint32_t)((int)cell + nb_offset(w, cx(i), cy(i), cz(i)));
+    if ((tipo[nb] & TYPE_MASK) != SOLIDO) continue;
+    const double q = 2.0 * ((double)S::ld(pop_at<LB>(pop, i, cell, g)) + wd(i));
+    if (cx(i)) f[0] += cx(i) > 0 ? q : -q;
+    if (cy(i)) f[1] += cy(i) > 0 ? q : -q;
+    if (cz(i)) f[2] += cz(i) > 0 ? q : -q;
+  }
+}
+
+constexpr int kLinkBit = 5;
+static_assert(BODY_LINK == 1u << kLinkBit, "flag bit of BODY_LINK");
+
+// The type scan reads 16 cells per lane (one 16-byte load); link cells are
+// few, so most lanes only test the flags (the type array starts on an
+// allocation boundary, so the loads are aligned).  The cells past the last whole
+// 16-byte word are scanned one by one.  Each thread's cells and their order
+// are fixed by the launch shape, so the sum is deterministic.
+template <class S, int LB>
+__global__ __launch_bounds__(kBlock) void k_forces(const typename S::T* __restrict__ pop_,
+                                                   const uint8_t* __restrict__ tipo_, Geom g,
+                                                   double* __restrict__ partials) {
+  const GLOBAL typename S::T* pop = gptr(pop_);
+  const GLOBAL uint8_t* tipo = gptr(tipo_);
+  double f[3] = {0.0, 0.0, 0.0};
+  const uint32_t words = g.N >> 4;
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t wi = blockIdx.x * kBlock + threadIdx.x; wi < words; wi += stride) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 t4 = *reinterpret_cast<const GLOBAL u32x4*>(tipo + 16u * wi);
+    const uint32_t t[4] = {t4.x, t4.y, t4.z, t4.w};
+    uint32_t mask = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) mask |= ((t[k >> 2] >> (8 * (k & 3) + kLinkBit)) & 1u) << k;
+    while (mask) {
+      const uint32_t k = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      link_force<S, LB>(pop, tipo, g, 16u * wi + k, f);
     }
+  }
+  for (uint32_t cell = 16u * words + blockIdx.x * kBlock + threadIdx.x; cell < g.N;
+       cell += stride) {
+    if (tipo[cell] & BODY_LINK) link_force<S, LB>(pop, tipo, g, cell, f);
   }
   block_sum<3>(f, partials + (size_t)blockIdx.x * 3);
 }
