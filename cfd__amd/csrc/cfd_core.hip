@@ -58,6 +58,9 @@ struct cfd_solver {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void* pop[2] = {nullptr, nullptr};
   uint8_t* tipo8 = nullptr;
+  uint2* links = nullptr;     // body links of the current types, cell order
+  uint32_t nlinks = 0;
+  size_t link_cap = 0;
   float4* macros = nullptr;
   double* partials = nullptr;
   double* result = nullptr;   // device, 4 doubles
@@ -76,6 +79,7 @@ void release(cfd_solver* s) {
   for (void* p : s->pop)
     if (p) (void)hipFree(p);
   if (s->tipo8) (void)hipFree(s->tipo8);
+  if (s->links) (void)hipFree(s->links);
   if (s->macros) (void)hipFree(s->macros);
   if (s->partials) (void)hipFree(s->partials);
   if (s->result) (void)hipFree(s->result);
@@ -259,6 +263,41 @@ int cfd_configure(cfd_solver* s, const cfd_params* p) {
   return CFD_OK;
 }
 
+namespace {
+// The body-link list of the current types (momentum exchange reads it):
+// count per chunk, scan, then write the entries into a list grown as needed.
+int build_links(cfd_solver* s) {
+  const uint32_t nb = link_chunks(s->geom);
+  uint32_t* tmp = nullptr;  // counts[nb], offsets[nb + 1]
+  HIPCK(hipMalloc(&tmp, (2 * (size_t)nb + 1) * sizeof(uint32_t)));
+  uint32_t total = 0;
+  hipError_t e = launch_link_count(s->tipo8, s->geom, tmp, tmp + nb, s->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&total, tmp + 2 * (size_t)nb, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                       s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess && total > s->link_cap) {
+    if (s->links) (void)hipFree(s->links);
+    s->links = nullptr;
+    s->link_cap = 0;
+    s->nlinks = 0;
+    e = hipMalloc(&s->links, (size_t)total * sizeof(uint2));
+    if (e == hipSuccess) s->link_cap = total;
+  }
+  if (e == hipSuccess && total)
+    e = launch_link_write(s->tipo8, s->geom, tmp + nb, s->links, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) {
+    s->nlinks = 0;
+    return fail(e == hipErrorOutOfMemory ? CFD_E_OOM : CFD_E_HIP, "cfd_set_types: %s",
+                hipGetErrorString(e));
+  }
+  s->nlinks = total;
+  return CFD_OK;
+}
+}  // namespace
+
 int cfd_set_types(cfd_solver* s, const uint32_t* tipo, size_t n) {
   g_err.clear();
   if (!s || !tipo) return fail(CFD_E_ARG, "cfd_set_types: null argument");
@@ -272,7 +311,7 @@ int cfd_set_types(cfd_solver* s, const uint32_t* tipo, size_t n) {
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   (void)hipFree(d32);
   if (e != hipSuccess) return fail(CFD_E_HIP, "cfd_set_types: %s", hipGetErrorString(e));
-  return CFD_OK;
+  return build_links(s);
 }
 
 int cfd_init_equilibrium(cfd_solver* s) {
@@ -453,8 +492,8 @@ int cfd_step_timed(cfd_solver* s, int n, double* ms_out) {
 int cfd_forces(cfd_solver* s, double out_f[3]) {
   g_err.clear();
   if (!s || !out_f) return fail(CFD_E_ARG, "cfd_forces: null argument");
-  HIPCK(launch_forces(s->store, s->pop[s->front], s->tipo8, s->geom, s->partials, s->result,
-                      s->stream));
+  HIPCK(launch_forces(s->store, s->pop[s->front], s->links, s->nlinks, s->geom, s->partials,
+                      s->result, s->stream));
   HIPCK(hipMemcpyAsync(s->host_res, s->result, 3 * sizeof(double), hipMemcpyDeviceToHost,
                        s->stream));
   HIPCK(hipStreamSynchronize(s->stream));

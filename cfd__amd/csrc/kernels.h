@@ -74,11 +74,22 @@ hipError_t launch_init(Store sto, void* dst, const uint8_t* tipo8, const Geom& g
                        const StepParams& p, hipStream_t st);
 hipError_t launch_macros(Store sto, const void* src, float4* macros, const Geom& g,
                          hipStream_t st);
-// Momentum exchange (3 doubles) and mass (sum, non-finite count) reductions.
-// `partials` must hold kReduceGrid * 3 doubles; result lands in out[0..2].
+// Body links: one entry {cell, mask of directions i with a SOLIDO neighbour
+// at x + c_i} per BODY_LINK cell, in cell order.  Built in two launches:
+// launch_link_count fills counts[link_chunks] and offsets[link_chunks + 1]
+// (offsets[link_chunks] = number of entries), then launch_link_write writes
+// the entries into `links`.
+uint32_t link_chunks(const Geom& g);
+hipError_t launch_link_count(const uint8_t* tipo8, const Geom& g, uint32_t* counts,
+                             uint32_t* offsets, hipStream_t st);
+hipError_t launch_link_write(const uint8_t* tipo8, const Geom& g, const uint32_t* offsets,
+                             uint2* links, hipStream_t st);
+// Momentum exchange (3 doubles) over the link list and mass (sum, non-finite
+// count) reductions.  `partials` must hold kReduceGrid * 3 doubles; result
+// lands in out[0..2].
 constexpr int kReduceGrid = 1024;
-hipError_t launch_forces(Store sto, const void* pop, const uint8_t* tipo8, const Geom& g,
-                         double* partials, double* out, hipStream_t st);
+hipError_t launch_forces(Store sto, const void* pop, const uint2* links, uint32_t nlinks,
+                         const Geom& g, double* partials, double* out, hipStream_t st);
 hipError_t launch_mass(Store sto, const void* pop, const uint8_t* tipo8, const Geom& g,
                        double* partials, double* out, hipStream_t st);
 // Dense host layout [19][N] <-> device layout [19][S] (S >= N).

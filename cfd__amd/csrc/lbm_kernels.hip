@@ -364,60 +364,132 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* out) {
   }
 }
 
+// ------------------------------------------------------------------ body links
+
 // Momentum exchange (wgsl.js:332-396): for a FLUIDO cell x and a SOLIDO
 // neighbour at x + c_i, the link delivers c_i * 2 (g_i(x) + w_i) to the body
-// (the 2 w_i c_i term does not cancel per link: lattice.js:37-45).  Only
-// cells flagged BODY_LINK look at their neighbours; accumulation is float64.
-template <class S, int LB>
-__device__ __forceinline__ void link_force(const GLOBAL typename S::T* __restrict__ pop,
-                                           const GLOBAL uint8_t* __restrict__ tipo,
-                                           const Geom& g, uint32_t cell, double (&f)[3]) {
-  const Coord c = split(cell, g);
-  const Wrap w = wrap_offsets(c, g);
-#pragma unroll
-  for (int i = 1; i < Q; ++i) {
-    const uint32_t nb = (uint32_t)((int)cell + nb_offset(w, cx(i), cy(i), cz(i)));
-    if ((tipo[nb] & TYPE_MASK) != SOLIDO) continue;
-    const double q = 2.0 * ((double)S::ld(pop_at<LB>(pop, i, cell, g)) + wd(i));
-    if (cx(i)) f[0] += cx(i) > 0 ? q : -q;
-    if (cy(i)) f[1] += cy(i) > 0 ? q : -q;
-    if (cz(i)) f[2] += cz(i) > 0 ? q : -q;
-  }
-}
+// (the 2 w_i c_i term does not cancel per link: lattice.js:37-45);
+// accumulation is float64.  The links depend on the cell types only, so they
+// are listed once when the types are set: one entry per cell flagged
+// BODY_LINK, in cell order, holding the cell and a bit mask of the directions
+// i whose neighbour x + c_i is SOLIDO.  A force call then reads the list and
+// one population per link instead of scanning the whole type field.
+//
+// The list is built in three passes that keep cell order: count the link
+// cells of each 4096-cell chunk (16 per lane), scan the chunk counts, write
+// each chunk's entries from its offset.
 
+constexpr uint32_t kLinkCellsPerLane = 16;
+constexpr uint32_t kLinkChunk = kBlock * kLinkCellsPerLane;
 constexpr int kLinkBit = 5;
 static_assert(BODY_LINK == 1u << kLinkBit, "flag bit of BODY_LINK");
 
-// The type scan reads 16 cells per lane (one 16-byte load); link cells are
-// few, so most lanes only test the flags (the type array starts on an
-// allocation boundary, so the loads are aligned).  The cells past the last whole
-// 16-byte word are scanned one by one.  Each thread's cells and their order
-// are fixed by the launch shape, so the sum is deterministic.
-template <class S, int LB>
-__global__ __launch_bounds__(kBlock) void k_forces(const typename S::T* __restrict__ pop_,
-                                                   const uint8_t* __restrict__ tipo_, Geom g,
-                                                   double* __restrict__ partials) {
-  const GLOBAL typename S::T* pop = gptr(pop_);
-  const GLOBAL uint8_t* tipo = gptr(tipo_);
-  double f[3] = {0.0, 0.0, 0.0};
-  const uint32_t words = g.N >> 4;
-  const uint32_t stride = gridDim.x * kBlock;
-  for (uint32_t wi = blockIdx.x * kBlock + threadIdx.x; wi < words; wi += stride) {
+// 16-bit mask of the BODY_LINK cells among this lane's 16 cells.  A whole
+// 16-byte word is read with one aligned load (the type array starts on an
+// allocation boundary); the cells of a last partial word one by one.
+__device__ __forceinline__ uint32_t link_mask16(const GLOBAL uint8_t* tipo, uint32_t first,
+                                                uint32_t N) {
+  uint32_t mask = 0;
+  if (first + kLinkCellsPerLane <= N) {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-    const u32x4 t4 = *reinterpret_cast<const GLOBAL u32x4*>(tipo + 16u * wi);
+    const u32x4 t4 = *reinterpret_cast<const GLOBAL u32x4*>(tipo + first);
     const uint32_t t[4] = {t4.x, t4.y, t4.z, t4.w};
-    uint32_t mask = 0;
 #pragma unroll
     for (int k = 0; k < 16; ++k) mask |= ((t[k >> 2] >> (8 * (k & 3) + kLinkBit)) & 1u) << k;
-    while (mask) {
-      const uint32_t k = __builtin_ctz(mask);
-      mask &= mask - 1u;
-      link_force<S, LB>(pop, tipo, g, 16u * wi + k, f);
-    }
+  } else {
+    for (uint32_t k = 0; first + k < N; ++k) mask |= ((tipo[first + k] >> kLinkBit) & 1u) << k;
   }
-  for (uint32_t cell = 16u * words + blockIdx.x * kBlock + threadIdx.x; cell < g.N;
-       cell += stride) {
-    if (tipo[cell] & BODY_LINK) link_force<S, LB>(pop, tipo, g, cell, f);
+  return mask;
+}
+
+// Exclusive prefix sum over the workgroup (wave64 shuffles, then the four
+// wave totals through LDS); *total receives the workgroup sum.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += o;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    if (w < wave) before += wsum[w];
+    all += wsum[w];
+  }
+  __syncthreads();
+  *total = all;
+  return before + inc - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_link_count(const uint8_t* __restrict__ tipo_, Geom g,
+                                                       uint32_t* __restrict__ counts) {
+  const GLOBAL uint8_t* tipo = gptr(tipo_);
+  const uint32_t first = blockIdx.x * kLinkChunk + threadIdx.x * kLinkCellsPerLane;
+  const uint32_t c = first < g.N ? __popc(link_mask16(tipo, first, g.N)) : 0u;
+  uint32_t total;
+  (void)block_exclusive_scan(c, &total);
+  if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+// One workgroup: offsets[k] = sum of counts[0..k), offsets[n] = total.
+__global__ __launch_bounds__(kBlock) void k_link_scan(const uint32_t* __restrict__ counts,
+                                                      uint32_t n, uint32_t* __restrict__ offsets) {
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n; base += kBlock) {
+    const uint32_t k = base + threadIdx.x;
+    const uint32_t v = k < n ? counts[k] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(v, &total);
+    if (k < n) offsets[k] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) offsets[n] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_link_write(const uint8_t* __restrict__ tipo_, Geom g,
+                                                       const uint32_t* __restrict__ offsets,
+                                                       uint2* __restrict__ links) {
+  const GLOBAL uint8_t* tipo = gptr(tipo_);
+  const uint32_t first = blockIdx.x * kLinkChunk + threadIdx.x * kLinkCellsPerLane;
+  uint32_t mask = first < g.N ? link_mask16(tipo, first, g.N) : 0u;
+  uint32_t total;
+  uint32_t pos = offsets[blockIdx.x] + block_exclusive_scan(__popc(mask), &total);
+  while (mask) {
+    const uint32_t cell = first + __builtin_ctz(mask);
+    mask &= mask - 1u;
+    const Coord c = split(cell, g);
+    const Wrap w = wrap_offsets(c, g);
+    uint32_t dirs = 0;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const uint32_t nb = (uint32_t)((int)cell + nb_offset(w, cx(i), cy(i), cz(i)));
+      if ((tipo[nb] & TYPE_MASK) == SOLIDO) dirs |= 1u << i;
+    }
+    links[pos++] = make_uint2(cell, dirs);
+  }
+}
+
+template <class S, int LB>
+__global__ __launch_bounds__(kBlock) void k_forces(const typename S::T* __restrict__ pop_,
+                                                   const uint2* __restrict__ links, uint32_t n,
+                                                   Geom g, double* __restrict__ partials) {
+  const GLOBAL typename S::T* pop = gptr(pop_);
+  double f[3] = {0.0, 0.0, 0.0};
+  for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
+    const uint2 L = links[e];
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      if (!((L.y >> i) & 1u)) continue;
+      const double q = 2.0 * ((double)S::ld(pop_at<LB>(pop, i, L.x, g)) + wd(i));
+      if (cx(i)) f[0] += cx(i) > 0 ? q : -q;
+      if (cy(i)) f[1] += cy(i) > 0 ? q : -q;
+      if (cz(i)) f[2] += cz(i) > 0 ? q : -q;
+    }
   }
   block_sum<3>(f, partials + (size_t)blockIdx.x * 3);
 }
@@ -599,14 +671,33 @@ static int reduce_grid(const Geom& g) {
   return b < (uint32_t)kReduceGrid ? (int)b : kReduceGrid;
 }
 
-hipError_t launch_forces(Store sto, const void* pop, const uint8_t* tipo8, const Geom& g,
-                         double* partials, double* out, hipStream_t st) {
-  const int gb = reduce_grid(g);
+uint32_t link_chunks(const Geom& g) { return (g.N + kLinkChunk - 1) / kLinkChunk; }
+
+hipError_t launch_link_count(const uint8_t* tipo8, const Geom& g, uint32_t* counts,
+                             uint32_t* offsets, hipStream_t st) {
+  const uint32_t nb = link_chunks(g);
+  hipLaunchKernelGGL(k_link_count, dim3(nb), dim3(kBlock), 0, st, tipo8, g, counts);
+  hipLaunchKernelGGL(k_link_scan, dim3(1), dim3(kBlock), 0, st, counts, nb, offsets);
+  return hipGetLastError();
+}
+
+hipError_t launch_link_write(const uint8_t* tipo8, const Geom& g, const uint32_t* offsets,
+                             uint2* links, hipStream_t st) {
+  hipLaunchKernelGGL(k_link_write, dim3(link_chunks(g)), dim3(kBlock), 0, st, tipo8, g,
+                     offsets, links);
+  return hipGetLastError();
+}
+
+hipError_t launch_forces(Store sto, const void* pop, const uint2* links, uint32_t nlinks,
+                         const Geom& g, double* partials, double* out, hipStream_t st) {
+  uint32_t gb = (nlinks + kBlock - 1) / kBlock;
+  if (gb < 1) gb = 1;
+  if (gb > (uint32_t)kReduceGrid) gb = kReduceGrid;
   CFD_DISPATCH(sto, g, {
     hipLaunchKernelGGL((k_forces<S, L>), dim3(gb), dim3(kBlock), 0, st,
-                       static_cast<const typename S::T*>(pop), tipo8, g, partials);
+                       static_cast<const typename S::T*>(pop), links, nlinks, g, partials);
   });
-  hipLaunchKernelGGL(k_reduce<3>, dim3(1), dim3(kBlock), 0, st, partials, gb, out);
+  hipLaunchKernelGGL(k_reduce<3>, dim3(1), dim3(kBlock), 0, st, partials, (int)gb, out);
   return hipGetLastError();
 }
 

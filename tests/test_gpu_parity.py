@@ -210,6 +210,47 @@ def test_rest_offset_closed_form():
         s.destruir()
 
 
+def test_force_link_list_follows_the_types(orc):
+    """The body links are listed when the types are set: re-setting the types
+    (a larger body, then a smaller one, then none) must change the force like
+    the oracle's scan over the type field does.  Periodic fluid around the
+    bodies; the grid (31x21x17 = 11 067 cells) is not a multiple of the 16
+    cells a lane scans, and the second field puts a solid bar in the last row,
+    so link cells fall in the final partial word."""
+    S = _solver()
+    grid = (31, 21, 17)
+    nx, ny, nz = grid
+    g = {"nx": nx, "ny": ny, "nz": nz}
+    pops = _random_state(grid, 5)
+
+    def field(body, tail=False):
+        t = np.zeros((nz, ny, nx), dtype=np.uint32)
+        if body:
+            t.reshape(-1)[tunel.corpoAnalitico(g, body).reshape(-1) != 0] = 1
+        if tail:
+            t[nz - 1, ny - 1, 24:28] = 1
+        return t.reshape(-1)
+
+    fields = [
+        field({"tipo": "esfera", "cx": 10, "cy": 10, "cz": 8, "raio": 3}),
+        field({"tipo": "esfera", "cx": 20, "cy": 11, "cz": 9, "raio": 6}, tail=True),
+        field({"tipo": "esfera", "cx": 8, "cy": 6, "cz": 5, "raio": 2}),
+        field(None),
+    ]
+    s = S.criar(g)
+    try:
+        for k, tipo in enumerate(fields):
+            s.definirTipos(tipo)
+            s.definirPopulacoes(pops)
+            fg = s.forcaBruta()
+            fo = orc.forces(grid, pops, tipo)
+            scale = max(np.abs(fo).max(), 1e-12)
+            assert np.abs(fg - fo).max() <= 1e-9 * scale + 1e-12, (k, fg, fo)
+        assert np.all(fg == 0.0)
+    finally:
+        s.destruir()
+
+
 def test_layouts_and_launch_shapes_are_bit_identical():
     """The population layout (SoA or 512-cell blocks) and the launch shape
     (workgroup size, XCD chunking) change addressing only: the populations
