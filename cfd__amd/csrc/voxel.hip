@@ -177,18 +177,23 @@ __global__ __launch_bounds__(kRasterBlock) void k_raster(
       if (threadIdx.x == 0) atomicAdd(outside, 1ull);
       continue;
     }
+    // the box lies inside the grid, whose cell count is below 2^31: 32-bit
+    // index arithmetic throughout
     const int x0 = (int)lo[0], y0 = (int)lo[1], z0 = (int)lo[2];
-    const int bx = (int)hi[0] - x0 + 1, by = (int)hi[1] - y0 + 1, bz = (int)hi[2] - z0 + 1;
-    const size_t ncell = (size_t)bx * by * bz;
-    for (size_t k = threadIdx.x; k < ncell; k += kRasterBlock) {
-      const int x = x0 + (int)(k % bx);
-      const size_t r = k / bx;
-      const int y = y0 + (int)(r % by);
-      const int z = z0 + (int)(r / by);
+    const uint32_t bx = (uint32_t)((int)hi[0] - x0 + 1), by = (uint32_t)((int)hi[1] - y0 + 1),
+                   bz = (uint32_t)((int)hi[2] - z0 + 1);
+    const uint32_t ncell = bx * by * bz;
+    for (uint32_t k = threadIdx.x; k < ncell; k += kRasterBlock) {
+      const uint32_t r = k / bx;
+      const int x = x0 + (int)(k - r * bx);
+      const uint32_t zr = r / by;
+      const int y = y0 + (int)(r - zr * by);
+      const int z = z0 + (int)zr;
       const double p[3] = {x + 0.5, y + 0.5, z + 0.5};
-      const size_t cell = ((size_t)z * ny + y) * nx + x;
+      const uint32_t cell = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx + (uint32_t)x;
       const double d2 = dist2_point_tri(p, a, b, c);
-      if (d2 == d2) atomicMin(&dist2bits[cell], __float_as_uint((float)d2));
+      // the distance field is kept only when a signed distance is asked for
+      if (dist2bits && d2 == d2) atomicMin(&dist2bits[cell], __float_as_uint((float)d2));
       if (d2 <= 0.75 && flags[cell] != CASCA) {
         const double la[3] = {a[0] - p[0], a[1] - p[1], a[2] - p[2]};
         const double lb[3] = {b[0] - p[0], b[1] - p[1], b[2] - p[2]};
@@ -200,35 +205,51 @@ __global__ __launch_bounds__(kRasterBlock) void k_raster(
 }
 
 constexpr int kCellBlock = 256;
+constexpr uint32_t kNoCell = 0xffffffffu;
+
+// This thread's cell of a one-thread-per-cell launch and its coordinates, in
+// 32-bit arithmetic (the grid holds fewer than 2^31 cells); kNoCell past the
+// end.
+__device__ __forceinline__ uint32_t cell_xyz(int nx, int ny, int nz, int* x, int* y, int* z) {
+  const uint32_t N = (uint32_t)nx * (uint32_t)ny * (uint32_t)nz;
+  const uint32_t cell = blockIdx.x * (uint32_t)kCellBlock + threadIdx.x;
+  if (cell >= N) return kNoCell;
+  const uint32_t r = cell / (uint32_t)nx;
+  *x = (int)(cell - r * (uint32_t)nx);
+  const uint32_t zz = r / (uint32_t)ny;
+  *y = (int)(r - zz * (uint32_t)ny);
+  *z = (int)zz;
+  return cell;
+}
 
 // 26-neighbour dilation of the shell, clamped at the faces (voxel.js:244-267).
-__global__ __launch_bounds__(kCellBlock) void k_dilate(const uint8_t* __restrict__ flags,
-                                                       uint8_t* __restrict__ barrier, int nx,
-                                                       int ny, int nz) {
-  const size_t N = (size_t)nx * ny * nz;
-  const size_t cell = blockIdx.x * (size_t)kCellBlock + threadIdx.x;
-  if (cell >= N) return;
-  const int x = (int)(cell % nx);
-  const int y = (int)((cell / nx) % ny);
-  const int z = (int)(cell / ((size_t)nx * ny));
-  uint8_t hit = 0;
-  for (int k = z > 0 ? z - 1 : 0; k <= (z < nz - 1 ? z + 1 : nz - 1); ++k)
-    for (int j = y > 0 ? y - 1 : 0; j <= (y < ny - 1 ? y + 1 : ny - 1); ++j)
-      for (int i = x > 0 ? x - 1 : 0; i <= (x < nx - 1 ? x + 1 : nx - 1); ++i)
-        hit |= (flags[((size_t)k * ny + j) * nx + i] == CASCA);
-  barrier[cell] = hit;
+// The 3x3x3 box maximum is separable: one pass per axis, each reading the
+// cell and its two neighbours along that axis (clamped), the first pass
+// testing for CASCA.
+__global__ __launch_bounds__(kCellBlock) void k_dilate_axis(const uint8_t* __restrict__ in,
+                                                            uint8_t* __restrict__ out, int nx,
+                                                            int ny, int nz, int axis,
+                                                            int shell) {
+  int x, y, z;
+  const uint32_t cell = cell_xyz(nx, ny, nz, &x, &y, &z);
+  if (cell == kNoCell) return;
+  const int pos = axis == 0 ? x : (axis == 1 ? y : z);
+  const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  const uint32_t step = axis == 0 ? 1u : (axis == 1 ? (uint32_t)nx : (uint32_t)nx * ny);
+  auto hit = [&](uint32_t c) -> uint8_t { return shell ? (in[c] == CASCA) : (in[c] != 0); };
+  uint8_t v = hit(cell);
+  if (pos > 0) v |= hit(cell - step);
+  if (pos < n - 1) v |= hit(cell + step);
+  out[cell] = v;
 }
 
 // Seed: every face cell that is not a barrier is exterior (voxel.js:291-309).
 __global__ __launch_bounds__(kCellBlock) void k_seed(const uint8_t* __restrict__ barrier,
                                                      uint8_t* __restrict__ ext, int nx, int ny,
                                                      int nz) {
-  const size_t N = (size_t)nx * ny * nz;
-  const size_t cell = blockIdx.x * (size_t)kCellBlock + threadIdx.x;
-  if (cell >= N) return;
-  const int x = (int)(cell % nx);
-  const int y = (int)((cell / nx) % ny);
-  const int z = (int)(cell / ((size_t)nx * ny));
+  int x, y, z;
+  const uint32_t cell = cell_xyz(nx, ny, nz, &x, &y, &z);
+  if (cell == kNoCell) return;
   const bool face = x == 0 || x == nx - 1 || y == 0 || y == ny - 1 || z == 0 || z == nz - 1;
   ext[cell] = (face && !barrier[cell]) ? 1 : 0;
 }
@@ -239,15 +260,30 @@ __global__ __launch_bounds__(kCellBlock) void k_seed(const uint8_t* __restrict__
 constexpr int TX = 32, TY = 8, TZ = 4;
 constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
 
+// A sweep works only on tiles that may change: those with a face neighbour
+// (or themselves) changed in the previous sweep, whose halo may then hold new
+// exterior cells (`prev`: one byte per tile, all ones for the first sweep).
+// A tile that no neighbour changed saw every exterior cell of its halo
+// already, so it is at its fixpoint.  `next` records this sweep's changes.
 __global__ __launch_bounds__(kCellBlock) void k_flood_tile(const uint8_t* __restrict__ barrier,
                                                            uint8_t* __restrict__ ext, int nx,
                                                            int ny, int nz,
+                                                           const uint8_t* __restrict__ prev,
+                                                           uint8_t* __restrict__ next,
                                                            int* __restrict__ changed) {
   __shared__ uint8_t st[HZ][HY][HX];
-  const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
+  const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY, gz = (nz + TZ - 1) / TZ;
   const int bxi = blockIdx.x % gx;
   const int byi = (blockIdx.x / gx) % gy;
   const int bzi = blockIdx.x / (gx * gy);
+  const int t = blockIdx.x;
+  const bool active = prev[t] || (bxi > 0 && prev[t - 1]) || (bxi < gx - 1 && prev[t + 1]) ||
+                      (byi > 0 && prev[t - gx]) || (byi < gy - 1 && prev[t + gx]) ||
+                      (bzi > 0 && prev[t - gx * gy]) || (bzi < gz - 1 && prev[t + gx * gy]);
+  if (!active) {  // uniform over the workgroup
+    if (threadIdx.x == 0) next[t] = 0;
+    return;
+  }
   const int ox = bxi * TX - 1, oy = byi * TY - 1, oz = bzi * TZ - 1;
   for (int k = threadIdx.x; k < HX * HY * HZ; k += kCellBlock) {
     const int lx = k % HX, ly = (k / HX) % HY, lz = k / (HX * HY);
@@ -287,6 +323,8 @@ __global__ __launch_bounds__(kCellBlock) void k_flood_tile(const uint8_t* __rest
     }
     *changed = 1;
   }
+  const int any = __syncthreads_or(owned_new != 0);
+  if (threadIdx.x == 0) next[t] = any ? 1 : 0;
 }
 
 // Skin recovery from a snapshot (voxel.js:333-355).
@@ -294,13 +332,10 @@ __global__ __launch_bounds__(kCellBlock) void k_skin(const uint8_t* __restrict__
                                                      const uint8_t* __restrict__ flags,
                                                      uint8_t* __restrict__ after, int nx,
                                                      int ny, int nz) {
-  const size_t N = (size_t)nx * ny * nz;
-  const size_t c = blockIdx.x * (size_t)kCellBlock + threadIdx.x;
-  if (c >= N) return;
-  const int x = (int)(c % nx);
-  const int y = (int)((c / nx) % ny);
-  const int z = (int)(c / ((size_t)nx * ny));
-  const size_t nxny = (size_t)nx * ny;
+  int x, y, z;
+  const uint32_t c = cell_xyz(nx, ny, nz, &x, &y, &z);
+  if (c == kNoCell) return;
+  const uint32_t nxny = (uint32_t)nx * (uint32_t)ny;
   uint8_t v = before[c];
   if (!v && flags[c] != CASCA) {
     v = (x > 0 && before[c - 1]) || (x < nx - 1 && before[c + 1]) ||
@@ -315,16 +350,16 @@ __global__ __launch_bounds__(kCellBlock) void k_final(
     const uint8_t* __restrict__ flags, const uint8_t* __restrict__ ext,
     const uint32_t* __restrict__ dist2bits, size_t N, double band, uint8_t* __restrict__ tipo,
     float* __restrict__ sdf, unsigned long long* __restrict__ counts) {
-  const size_t c = blockIdx.x * (size_t)kCellBlock + threadIdx.x;
   unsigned long long casca = 0, dentro = 0;
-  if (c < N) {
+  for (size_t c = blockIdx.x * (size_t)kCellBlock + threadIdx.x; c < N;
+       c += (size_t)gridDim.x * kCellBlock) {
     uint8_t t = 0;
     if (flags[c] == CASCA) {
       t = 1;
-      casca = 1;
+      ++casca;
     } else if (!ext[c]) {
       t = 2;
-      dentro = 1;
+      ++dentro;
     }
     tipo[c] = t;
     if (sdf) {
@@ -333,14 +368,23 @@ __global__ __launch_bounds__(kCellBlock) void k_final(
       sdf[c] = (float)(t == 0 ? d : -d);
     }
   }
-  // wave-aggregated counters
+  // workgroup-aggregated counters: one atomic per workgroup and counter (the
+  // grid is capped, so few atomics meet at the two addresses)
   for (int off = 32; off > 0; off >>= 1) {
     casca += __shfl_xor(casca, off, 64);
     dentro += __shfl_xor(dentro, off, 64);
   }
-  if ((threadIdx.x & 63) == 0) {
-    if (casca) atomicAdd(&counts[0], casca);
-    if (dentro) atomicAdd(&counts[1], dentro);
+  __shared__ unsigned long long part[2][kCellBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    part[0][wave] = casca;
+    part[1][wave] = dentro;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    unsigned long long v = 0;
+    for (int w = 0; w < kCellBlock / 64; ++w) v += part[threadIdx.x][w];
+    if (v) atomicAdd(&counts[threadIdx.x], v);
   }
 }
 
@@ -401,9 +445,12 @@ extern "C" int cfd_voxelize(const float* pos, size_t nv, const uint32_t* idx, si
   float* sdf = nullptr;
   unsigned long long* counters = nullptr;  // [0] casca [1] dentro [2] outside
   int* changed = nullptr;
+  uint8_t* tile_chg = nullptr;  // two sweeps' per-tile change flags
   unsigned long long hc[3] = {0, 0, 0};
   int iters = 0;
   const unsigned cgrid = (unsigned)((N + kCellBlock - 1) / kCellBlock);
+  const unsigned tgrid =
+      (unsigned)(((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * ((nz + TZ - 1) / TZ));
 
   VCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   for (auto& e : ev) VCK(hipEventCreate(&e));
@@ -414,16 +461,21 @@ extern "C" int cfd_voxelize(const float* pos, size_t nv, const uint32_t* idx, si
   VCK(hipMalloc(&ext, N));
   VCK(hipMalloc(&ext2, N));
   VCK(hipMalloc(&tipo, N));
-  VCK(hipMalloc(&dist2, N * sizeof(uint32_t)));
+  // the float32 distance field (atomic minimum per cell) only feeds the
+  // signed distance; without sdf_out the rasterizer skips it
+  if (sdf_out) VCK(hipMalloc(&dist2, N * sizeof(uint32_t)));
   if (sdf_out) VCK(hipMalloc(&sdf, N * sizeof(float)));
   VCK(hipMalloc(&counters, 3 * sizeof(unsigned long long)));
   VCK(hipMalloc(&changed, sizeof(int)));
+  VCK(hipMalloc(&tile_chg, 2 * (size_t)tgrid));
   if (nv) VCK(hipMemcpyAsync(d_pos, pos, nv * 3 * sizeof(float), hipMemcpyHostToDevice, st));
   if (nt) VCK(hipMemcpyAsync(d_idx, idx, nt * 3 * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   VCK(hipMemsetAsync(flags, 0, N, st));
   VCK(hipMemsetAsync(counters, 0, 3 * sizeof(unsigned long long), st));
-  hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, st, dist2, N, INF_BITS);
-  VCK(hipGetLastError());
+  if (dist2) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, st, dist2, N, INF_BITS);
+    VCK(hipGetLastError());
+  }
 
   VCK(hipEventRecord(ev[0], st));
   if (nt) {
@@ -437,19 +489,26 @@ extern "C" int cfd_voxelize(const float* pos, size_t nv, const uint32_t* idx, si
   {
     const uint8_t* bar = flags;
     if (seal) {
-      hipLaunchKernelGGL(k_dilate, dim3(cgrid), dim3(kCellBlock), 0, st, flags, barrier, nx,
-                         ny, nz);
+      // x pass into ext, y pass into ext2 (both rewritten before use), z pass
+      // into the barrier
+      hipLaunchKernelGGL(k_dilate_axis, dim3(cgrid), dim3(kCellBlock), 0, st, flags, ext, nx,
+                         ny, nz, 0, 1);
+      hipLaunchKernelGGL(k_dilate_axis, dim3(cgrid), dim3(kCellBlock), 0, st, ext, ext2, nx,
+                         ny, nz, 1, 0);
+      hipLaunchKernelGGL(k_dilate_axis, dim3(cgrid), dim3(kCellBlock), 0, st, ext2, barrier, nx,
+                         ny, nz, 2, 0);
       VCK(hipGetLastError());
       bar = barrier;
     }
     hipLaunchKernelGGL(k_seed, dim3(cgrid), dim3(kCellBlock), 0, st, bar, ext, nx, ny, nz);
     VCK(hipGetLastError());
-    const unsigned tgrid =
-        (unsigned)(((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * ((nz + TZ - 1) / TZ));
+    VCK(hipMemsetAsync(tile_chg, 1, tgrid, st));   // first sweep: every tile
     for (;;) {
       VCK(hipMemsetAsync(changed, 0, sizeof(int), st));
+      const uint8_t* prev = tile_chg + (size_t)(iters & 1) * tgrid;
+      uint8_t* next = tile_chg + (size_t)((iters + 1) & 1) * tgrid;
       hipLaunchKernelGGL(k_flood_tile, dim3(tgrid), dim3(kCellBlock), 0, st, bar, ext, nx, ny,
-                         nz, changed);
+                         nz, prev, next, changed);
       VCK(hipGetLastError());
       int hchg = 0;
       VCK(hipMemcpyAsync(&hchg, changed, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -465,8 +524,8 @@ extern "C" int cfd_voxelize(const float* pos, size_t nv, const uint32_t* idx, si
       ext_final = ext2;
     }
     VCK(hipEventRecord(ev[2], st));
-    hipLaunchKernelGGL(k_final, dim3(cgrid), dim3(kCellBlock), 0, st, flags, ext_final, dist2,
-                       N, band, tipo, sdf, counters);
+    hipLaunchKernelGGL(k_final, dim3(cgrid < 2048u ? cgrid : 2048u), dim3(kCellBlock), 0, st,
+                       flags, ext_final, dist2, N, band, tipo, sdf, counters);
     VCK(hipGetLastError());
   }
   VCK(hipEventRecord(ev[3], st));
@@ -495,7 +554,7 @@ done:
   if (st) (void)hipStreamSynchronize(st);
   for (void* p : {(void*)d_pos, (void*)d_idx, (void*)flags, (void*)barrier, (void*)ext,
                   (void*)ext2, (void*)tipo, (void*)dist2, (void*)sdf, (void*)counters,
-                  (void*)changed})
+                  (void*)changed, (void*)tile_chg})
     if (p) (void)hipFree(p);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);

@@ -187,12 +187,12 @@ def main():
         tot.append(st["msTotal"])
         its.append(st["iteracoesInundacao"])
     pares = pares_raster(pos, idx, GRID, banda)
-    # flood-side bytes per voxelization: dilate (read flags, write barrier),
-    # seed (read barrier, write ext), each tile sweep (read barrier + ext, write
-    # ext), skin (read before + flags, write after), final (read flags + ext,
-    # write tipo); dist2 (4 B) filled once and read by nothing when sdf is off
+    # flood-side bytes per voxelization: three dilation passes (read one
+    # array, write one), seed (read barrier, write ext), each tile sweep (read
+    # barrier + ext, write ext), skin (read before + flags, write after), final
+    # (read flags + ext, write tipo); no distance field without the sdf
     sweeps = float(np.median(its))
-    bytes_flood = N * (2 + 2 + 3 * sweeps + 3 + 3) + 4 * N
+    bytes_flood = N * (3 * 2 + 2 + 3 * sweeps + 3 + 3)
 
     tipo, esponja, _ = tunel.montarTipos(g, v["tipo"], esteira=False)
     link_cells, links = contar_links(tipo, GRID)

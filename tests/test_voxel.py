@@ -142,6 +142,14 @@ def test_gpu_voxelizer_bit_exact(name):
     s = r["stats"]
     assert (s["nCasca"], s["nDentro"], s["trianguloFora"], s["ocoSuspeito"]) == \
         (g["nCasca"], g["nDentro"], g["trianguloFora"], g["ocoSuspeito"])
+    # without the signed distance the rasterizer keeps no distance field:
+    # the types and counts must not change
+    r2 = voxelizar(pos, idx, {"nx": nx, "ny": ny, "nz": nz}, banda=banda, selarFrestas=selar,
+                   comSdf=False)
+    assert r2["sdf"] is None and _sha(r2["tipo"]) == g["tipo_sha256"], name
+    s2 = r2["stats"]
+    assert (s2["nCasca"], s2["nDentro"], s2["trianguloFora"]) == \
+        (g["nCasca"], g["nDentro"], g["trianguloFora"])
 
 
 @pytest.mark.gpu
