@@ -129,26 +129,31 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   const Wrap w = wrap_offsets(c, g);
 
   // pull with halfway bounce-back (ir.js:166-189), as in step_cell
+  constexpr int kSz = (int)sizeof(typename S::T);
   const uint32_t tb = tipo[cell];
+  const uint32_t cb = cell * (uint32_t)kSz;
+  const Wrap wb = scaled(w, kSz);
+  const uint32_t own = cell_byte_off<LB, typename S::T>(cb);
+  const PopView<S, LB> in(src, g), out(dst, g);
   float gi[Q];
-  uint32_t sidx[Q];
-  gi[0] = S::ld(pop_at<LB>(src, 0, cell, g));
+  uint32_t sb[Q];
+  gi[0] = in.ld(0, own);
 #pragma unroll
   for (int i = 1; i < Q; ++i) {
-    sidx[i] = (uint32_t)((int)cell + nb_offset(w, -cx(i), -cy(i), -cz(i)));
-    gi[i] = S::ld(pop_at<LB>(src, i, sidx[i], g));
+    sb[i] = cb + (uint32_t)nb_offset(wb, -cx(i), -cy(i), -cz(i));
+    gi[i] = in.ld(i, cell_byte_off<LB, typename S::T>(sb[i]));
   }
   const uint32_t ct = tb & TYPE_MASK;
   const bool wall = is_wall(ct);
   if (!wall && (tb & NEAR_WALL)) {
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
-      const uint32_t t = tipo[sidx[i]] & TYPE_MASK;
+      const uint32_t t = tipo[sb[i] / (uint32_t)kSz] & TYPE_MASK;
       if (is_wall(t)) {
         const bool moving = (t == SOLIDO_MOVEL);
         const float cu = dotc(i, moving ? p.belt[0] : 0.f, moving ? p.belt[1] : 0.f,
                               moving ? p.belt[2] : 0.f);
-        gi[i] = S::ld(pop_at<LB>(src, opp(i), cell, g)) + kwall(i) * cu;
+        gi[i] = in.ld(opp(i), own) + kwall(i) * cu;
       }
     }
   }
@@ -204,7 +209,7 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
     const uint32_t h = S::kStochastic ? sr_hash(cell, p.sr_seed) : 0u;
 #pragma unroll
     for (int i = 0; i < Q; ++i)
-      store_pop<S>(pop_at<LB>(dst, i, cell, g), wall ? 0.0f : d[i], h, i);
+      out.st(i, own, wall ? 0.0f : d[i], h);
   }
   if (MACROS && live)
     macros[cell] = wall ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(M.ux, M.uy, M.uz, M.delta);

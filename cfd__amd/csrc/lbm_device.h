@@ -34,10 +34,13 @@ __device__ __forceinline__ GLOBAL T* gptr(T* p) {
 struct StoreF32 {
   static constexpr bool kStochastic = false;
   using T = float;
+  using Bits = uint32_t;
   template <class P>
   static __device__ __forceinline__ float ld(P p) { return *p; }
   template <class P>
   static __device__ __forceinline__ void st(P p, float v) { *p = v; }
+  static __device__ __forceinline__ float dec(Bits b) { return __uint_as_float(b); }
+  static __device__ __forceinline__ Bits enc(float v) { return __float_as_uint(v); }
 };
 
 // float16 deviations g_i = f_i - w_i scaled by 2^12 (exact power of two, so
@@ -52,6 +55,13 @@ struct StoreF16 {
   }
   template <class P>
   static __device__ __forceinline__ void st(P p, float v) { *p = static_cast<T>(v * 4096.0f); }
+  using Bits = uint16_t;
+  static __device__ __forceinline__ float dec(Bits b) {
+    return static_cast<float>(__builtin_bit_cast(T, b)) * 0.000244140625f;
+  }
+  static __device__ __forceinline__ Bits enc(float v) {
+    return __builtin_bit_cast(Bits, static_cast<T>(v * 4096.0f));
+  }
 };
 
 // The same float16 storage with stochastic rounding in the step's stores: the
@@ -64,12 +74,18 @@ struct StoreF16 {
 // conversions are those of StoreF16.
 struct StoreF16S : StoreF16 {
   static constexpr bool kStochastic = true;
-  template <class P>
-  static __device__ __forceinline__ void st_sr(P p, float v, uint32_t bits13) {
+  static __device__ __forceinline__ T round_sr(float v, uint32_t bits13) {
     uint32_t u = __float_as_uint(v * 4096.0f);
     const uint32_t e = (u >> 23) & 0xffu;
     if (e - 113u < 29u) u = (u + bits13) & 0xffffe000u;   // 2^-14 <= |x| < 2^15
-    *p = static_cast<T>(__uint_as_float(u));
+    return static_cast<T>(__uint_as_float(u));
+  }
+  template <class P>
+  static __device__ __forceinline__ void st_sr(P p, float v, uint32_t bits13) {
+    *p = round_sr(v, bits13);
+  }
+  static __device__ __forceinline__ Bits enc_sr(float v, uint32_t bits13) {
+    return __builtin_bit_cast(Bits, round_sr(v, bits13));
   }
 };
 
@@ -136,6 +152,86 @@ __device__ __forceinline__ T* pop_at(T* base, int i, uint32_t c, const Geom& g) 
   }
 }
 
+// Byte addressing of the step's 19 loads and 19 stores.  A population's
+// address is a direction base (wave-uniform: i * S elements for SoA, i * 2^LB
+// for blocks) plus a per-lane byte offset of the cell.  With blocks,
+// the element index (s >> LB) * Q * 2^LB + i * 2^LB + (s & (2^LB - 1)) equals
+// s + (Q - 1) * 2^LB * (s >> LB) + i * 2^LB, so from the cell's byte index
+// sb = s * sizeof(T) the offset is one shift and one 24-bit multiply-add
+// (s >> LB < 2^21 for a handle's fewer than 2^30 cells; the product stays
+// below 2^32 because blocks are only used for sets below 4 GiB).
+template <int LB, class T>
+__device__ __forceinline__ uint32_t cell_byte_off(uint32_t sb) {
+  if constexpr (LB == 0) {
+    return sb;
+  } else {
+    constexpr uint32_t kLog = sizeof(T) == 4 ? 2u : (sizeof(T) == 2 ? 1u : 0u);
+    static_assert((1u << kLog) == sizeof(T), "2- or 4-byte populations");
+    constexpr uint32_t kStride = (uint32_t)(Q - 1) * (1u << LB) * (uint32_t)sizeof(T);
+    return __umul24(sb >> (LB + kLog), kStride) + sb;
+  }
+}
+template <class T>
+__device__ __forceinline__ T* at_byte(T* base, uint32_t off) {
+  return (T*)((GLOBAL char*)base + off);
+}
+template <class T>
+__device__ __forceinline__ const T* at_byte(const T* base, uint32_t off) {
+  return (const T*)((const GLOBAL char*)base + off);
+}
+
+// The step's view of a population set.  Blocked sets (LB > 0, below 4 GiB)
+// are read and written with raw buffer instructions: one resource for the
+// set, the direction's base as the scalar offset and the cell's byte offset
+// as the only per-lane address (no 64-bit address arithmetic per access).
+// SoA sets (which may exceed 4 GiB) use a wave-uniform 64-bit base per
+// direction and the same 32-bit byte offset.
+constexpr int kBufferWord3 = 0x00020000;  // raw dword access, gfx9-family resource
+
+template <class S, int LB>
+struct PopView {
+  using T = typename S::T;
+  GLOBAL T* base;
+  size_t slice;  // SoA: elements between direction slices
+  __amdgpu_buffer_rsrc_t rsrc;
+
+  __device__ __forceinline__ PopView(const GLOBAL T* p, const Geom& g)
+      : base(const_cast<GLOBAL T*>(p)), slice(g.S) {
+    if constexpr (LB > 0)
+      rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)0xffffffff,
+                                               kBufferWord3);
+  }
+  static __device__ __forceinline__ int dir_off(int i) {
+    return i * (int)(1u << LB) * (int)sizeof(T);
+  }
+  __device__ __forceinline__ float ld(int i, uint32_t off) const {
+    if constexpr (LB == 0) {
+      return S::ld(at_byte(base + (size_t)i * slice, off));
+    } else if constexpr (sizeof(T) == 4) {
+      return S::dec((typename S::Bits)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)off,
+                                                                           dir_off(i), 0));
+    } else {
+      return S::dec((typename S::Bits)__builtin_amdgcn_raw_buffer_load_b16(rsrc, (int)off,
+                                                                           dir_off(i), 0));
+    }
+  }
+  // population i of the cell at byte offset off: rounded to nearest, or
+  // stochastically for StoreF16S (h = sr_hash of the cell for this step)
+  __device__ __forceinline__ void st(int i, uint32_t off, float v, uint32_t h) const {
+    if constexpr (LB == 0) {
+      store_pop<S>(at_byte(base + (size_t)i * slice, off), v, h, i);
+    } else {
+      typename S::Bits b;
+      if constexpr (S::kStochastic) b = S::enc_sr(v, sr_bits(h, i));
+      else b = S::enc(v);
+      if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32((int)b, rsrc, (int)off, dir_off(i), 0);
+      else
+        __builtin_amdgcn_raw_buffer_store_b16((short)b, rsrc, (int)off, dir_off(i), 0);
+    }
+  }
+};
+
 __device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, uint32_t s) {
   return (__umulhi(n, m) + n) >> s;
 }
@@ -186,6 +282,11 @@ __device__ __forceinline__ Wrap wrap_offsets(const Coord& c, const Geom& g) {
   w.zm = c.z == 0 ? (nz - 1) * nxny : -nxny;
   w.zp = c.z == g.nz - 1 ? -(nz - 1) * nxny : nxny;
   return w;
+}
+
+// The same offsets in bytes of a T-sized population.
+__device__ __forceinline__ Wrap scaled(const Wrap& w, int sz) {
+  return Wrap{w.xm * sz, w.xp * sz, w.ym * sz, w.yp * sz, w.zm * sz, w.zp * sz};
 }
 
 // Offset of the cell at x + (dx, dy, dz), each component in {-1, 0, 1}.

@@ -84,14 +84,22 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
   //    byte and all 19 streamed values are requested together: a cell away
   //    from walls costs one memory round trip.  Only cells flagged NEAR_WALL
   //    look at their neighbours' types and re-load the bounced directions.
+  //    Addresses are formed in bytes: the cell's byte index plus the
+  //    neighbour's byte offset, one shift and one multiply-add to the
+  //    cell's place in its block, and a wave-uniform base per direction.
+  constexpr int kSz = (int)sizeof(typename S::T);
   const uint32_t tb = tipo[cell];
+  const uint32_t cb = cell * (uint32_t)kSz;
+  const Wrap wb = scaled(w, kSz);
+  const uint32_t own = cell_byte_off<LB, typename S::T>(cb);
+  const PopView<S, LB> in(src, g), out(dst, g);
   float gi[Q];
-  uint32_t sidx[Q];
-  gi[0] = S::ld(pop_at<LB>(src, 0, cell, g));
+  uint32_t sb[Q];
+  gi[0] = in.ld(0, own);
 #pragma unroll
   for (int i = 1; i < Q; ++i) {
-    sidx[i] = (uint32_t)((int)cell + nb_offset(w, -cx(i), -cy(i), -cz(i)));
-    gi[i] = S::ld(pop_at<LB>(src, i, sidx[i], g));
+    sb[i] = cb + (uint32_t)nb_offset(wb, -cx(i), -cy(i), -cz(i));
+    gi[i] = in.ld(i, cell_byte_off<LB, typename S::T>(sb[i]));
   }
   const uint32_t ct = tb & TYPE_MASK;
 
@@ -106,19 +114,19 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
     if (MACROS) macros[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (mixed) {
 #pragma unroll
-      for (int i = 0; i < Q; ++i) S::st(pop_at<LB>(dst, i, cell, g), 0.0f);
+      for (int i = 0; i < Q; ++i) out.st(i, own, 0.0f, 0u);
     }
     return;
   }
   if (tb & NEAR_WALL) {
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
-      const uint32_t t = tipo[sidx[i]] & TYPE_MASK;
+      const uint32_t t = tipo[sb[i] / (uint32_t)kSz] & TYPE_MASK;
       if (is_wall(t)) {
         const bool moving = (t == SOLIDO_MOVEL);
         const float cu = dotc(i, moving ? p.belt[0] : 0.f, moving ? p.belt[1] : 0.f,
                               moving ? p.belt[2] : 0.f);
-        const float back = S::ld(pop_at<LB>(src, opp(i), cell, g));
+        const float back = in.ld(opp(i), own);
         gi[i] = back + kwall(i) * cu;
         if (FORCE && t == SOLIDO && ct == FLUIDO) {
           const int j = opp(i);
@@ -146,7 +154,7 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
 
   const uint32_t h = S::kStochastic ? sr_hash(cell, p.sr_seed) : 0u;
 #pragma unroll
-  for (int i = 0; i < Q; ++i) store_pop<S>(pop_at<LB>(dst, i, cell, g), d[i], h, i);
+  for (int i = 0; i < Q; ++i) out.st(i, own, d[i], h);
   if (MACROS) macros[cell] = make_float4(M.ux, M.uy, M.uz, M.delta);
 }
 
