@@ -146,3 +146,40 @@ def test_prepared_vehicle_voxelizes_like_the_benchmark_body():
     x0, x1 = solid[:, 2].min(), solid[:, 2].max()
     assert abs(x0 - math.floor(96 * 0.28)) <= 1 and abs((x1 - x0 + 1) - 40) <= 2
     assert tunel.areaFrontalCorpo(tipo, g) > 0
+
+
+def _cube_stl() -> bytes:
+    """The 12 triangles of examples/cube_example.stl as a standard binary STL
+    (80-byte header, zero normals; the reference file itself has a 79-byte
+    header, which test_voxel.py covers)."""
+    import struct
+    import sys
+    sys.path.insert(0, str(MESHES.parent))
+    from voxel_cases import CUBE_STL_TRIANGLES
+    tri = CUBE_STL_TRIANGLES.astype("<f4").reshape(-1, 9)
+    out = bytearray(b"cube".ljust(80, b" ")) + struct.pack("<I", len(tri))
+    for t in tri:
+        out += struct.pack("<3f", 0.0, 0.0, 0.0) + t.tobytes() + b"\0\0"
+    return bytes(out)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nome", ["cube_example.stl", "vehicle.stl", "vehicle.glb"])
+def test_geometry_page(nome):
+    """tests/geometria.html:89-160 on the meshes at hand (the page's car files
+    are not in the reference tree; nodes.glb is a transform fixture of four
+    loose tetrahedra, not a body): read, place on the 'baixa' preset
+    (240x120x96, body round(240/10) = 24 cells long), voxelize on the GPU;
+    the page's check is a non-degenerate solid: more than 200 solid cells, a
+    solid fraction below 0.35 and a frontal area above 50 cells."""
+    from cfd__amd import tunel
+    from cfd__amd.voxel import voxelizar
+    buf = _cube_stl() if nome == "cube_example.stl" else (MESHES / nome).read_bytes()
+    m = mesh_io.ler(buf, nome)
+    g = {"nx": 240, "ny": 120, "nz": 96}
+    p = preparo.preparar(m, g, {"celulasPorComprimento": round(240 / 10)})
+    v = voxelizar(p["positions"], p["indices"], g, comSdf=False)
+    s = v["stats"]
+    af = tunel.areaFrontal(v["tipo"], g)
+    assert s["nSolido"] > 200 and s["fracaoSolida"] < 0.35 and af > 50, (nome, s, af)
+    assert not s["ocoSuspeito"], (nome, s)
