@@ -15,7 +15,9 @@
 //  * one thread per cell over a flat index; the flat index is split into
 //    (x, y, z) with two multiply-high divisions;
 //  * the 19 pull loads of a cell are issued together with its type byte: one
-//    memory round trip for every cell away from walls;
+//    memory round trip for every cell away from walls; blocked sets are read
+//    and written with raw buffer instructions (one resource per set, the
+//    direction as scalar offset, one 32-bit byte offset per direction);
 //  * the cell type is one byte with two precomputed flags (near-wall, body
 //    link), so the 18 neighbour-type reads of the bounce-back test happen only
 //    for cells that touch a wall.
