@@ -22,6 +22,19 @@ namespace cfd {
 // addresses.
 #define GLOBAL __attribute__((address_space(1)))
 
+// Constant address space: tables that no kernel writes while it runs (the
+// batch schedule and tunnel descriptors).  Read through such a pointer at a
+// wave-uniform address, they come through scalar loads, whose waits do not
+// also wait for the vector stores in flight.
+#define CONSTANT __attribute__((address_space(4)))
+
+template <class T>
+__device__ __forceinline__ T cload(const T* p, size_t i) {
+  T v;
+  __builtin_memcpy(&v, (const CONSTANT T*)p + i, sizeof(T));
+  return v;
+}
+
 template <class T>
 __device__ __forceinline__ const GLOBAL T* gptr(const T* p) {
   return (const GLOBAL T*)p;

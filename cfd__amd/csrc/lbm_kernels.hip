@@ -226,19 +226,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
   __shared__ uint32_t s_tile[2];
   if (blockIdx.x == 0 && threadIdx.x == 0) L.clocks[0] = wall_clock64();
   for (int ph = 0; ph < L.nphases; ++ph) {
-    const BatchPhase P = L.phases[ph];
+    const BatchPhase P = cload(L.phases, ph);
     for (uint32_t r = P.r0; r < P.r1; ++r) {
       uint32_t* ctr = L.counters + r;
       const uint32_t grab = (uint32_t)L.grab;
       if (threadIdx.x == 0) s_tile[0] = atomicAdd(ctr, grab);
       __syncthreads();
-      uint32_t t0 = s_tile[0];
+      // the tile index is the same in every lane: made wave-uniform, the
+      // tile's table, entry and tunnel descriptor come through scalar loads,
+      // whose waits do not wait for the previous tile's vector stores
+      uint32_t t0 = __builtin_amdgcn_readfirstlane(s_tile[0]);
       for (uint32_t it = 1; t0 < P.total_tiles; ++it) {
         uint32_t next = 0;
         if (threadIdx.x == 0) next = atomicAdd(ctr, grab);  // consumed after the tiles
         const uint32_t tend = min(t0 + grab, P.total_tiles);
         for (uint32_t t = t0; t < tend; ++t) {
-        const BatchEntry E = L.entries[P.first + L.tile_entry[P.table + t]];
+        const BatchEntry E = cload(L.entries, P.first + cload(L.tile_entry, P.table + t));
         const BatchTunnel& D = tun[E.tunnel];
         const uint32_t step = r - E.start_round;
         const uint32_t from = (step + L.parity0) & 1u;
@@ -274,7 +277,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
         // ping-pong slot: a wave still reading the previous slot is not raced
         if (threadIdx.x == 0) s_tile[it & 1] = next;
         __syncthreads();
-        t0 = s_tile[it & 1];
+        t0 = __builtin_amdgcn_readfirstlane(s_tile[it & 1]);
       }
       grid.sync();
       if (blockIdx.x == 0 && threadIdx.x == 0) L.clocks[1 + r] = wall_clock64();
