@@ -219,7 +219,7 @@ __device__ __forceinline__ void block_sum_into(double (&v)[3], double (&out)[3])
 // Sampled steps also accumulate the body force (float64 per workgroup, one
 // atomic add per workgroup and component into the tunnel's sample slot).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_batch_run(const BatchTunnel* __restrict__ tun,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 512 ? 6 : 5))) void k_batch_run(const BatchTunnel* __restrict__ tun,
                                                      const BatchLaunch L) {
   namespace cg = cooperative_groups;
   cg::grid_group grid = cg::this_grid();
