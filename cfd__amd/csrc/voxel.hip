@@ -254,11 +254,40 @@ __global__ __launch_bounds__(kCellBlock) void k_seed(const uint8_t* __restrict__
   ext[cell] = (face && !barrier[cell]) ? 1 : 0;
 }
 
-// Tile flood.  LDS state per cell: 0 open, 1 exterior, 2 blocked (barrier or
-// outside the grid).  Interior cells are updated in place until the tile
-// stops changing; halo cells are the global state at load time.
-constexpr int TX = 32, TY = 8, TZ = 4;
-constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+// Tile flood, bit-parallel.  A tile is 32 x 16 x 16 cells; each of its 256
+// x-rows is one 64-bit word in LDS, bits 1..32 the row's cells and bits 0
+// and 33 its x-neighbours outside the tile, with a one-row halo in y and z.
+// Per row two words: E (exterior) and O (open: in the grid and not a
+// barrier).  One thread owns one row; an iteration takes the exterior bits of
+// the four y/z neighbour rows where the row is open, then fills the row along
+// x in both directions through its open runs (Kogge-Stone occluded fill:
+// six shift steps per direction).  The tile iterates until no row changes;
+// the exterior only grows, so reading a neighbour row while it is rewritten
+// sees an old or a new value, both subsets of the fixpoint, and the loop ends
+// only after an iteration in which nothing was written.  The fixpoint is
+// unique, so the result does not depend on the tile shape or the order.
+constexpr int FX = 32, FY = 16, FZ = 16;        // tile (x bits, y rows, z rows)
+constexpr int RY = FY + 2, RZ = FZ + 2;         // with the y/z halo
+constexpr uint64_t kInterior = 0x1FFFFFFFEull;  // bits 1..32
+static_assert(FY * FZ == kCellBlock, "one thread per interior row");
+
+__device__ __forceinline__ uint64_t fill_x(uint64_t gen, uint64_t pro) {
+  uint64_t g = gen, p = pro;
+  g |= p & (g << 1);  p &= p << 1;
+  g |= p & (g << 2);  p &= p << 2;
+  g |= p & (g << 4);  p &= p << 4;
+  g |= p & (g << 8);  p &= p << 8;
+  g |= p & (g << 16); p &= p << 16;
+  g |= p & (g << 32);
+  p = pro;
+  g |= p & (g >> 1);  p &= p >> 1;
+  g |= p & (g >> 2);  p &= p >> 2;
+  g |= p & (g >> 4);  p &= p >> 4;
+  g |= p & (g >> 8);  p &= p >> 8;
+  g |= p & (g >> 16); p &= p >> 16;
+  g |= p & (g >> 32);
+  return g;
+}
 
 // A sweep works only on tiles that may change: those with a face neighbour
 // (or themselves) changed in the previous sweep, whose halo may then hold new
@@ -271,8 +300,9 @@ __global__ __launch_bounds__(kCellBlock) void k_flood_tile(const uint8_t* __rest
                                                            const uint8_t* __restrict__ prev,
                                                            uint8_t* __restrict__ next,
                                                            int* __restrict__ changed) {
-  __shared__ uint8_t st[HZ][HY][HX];
-  const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY, gz = (nz + TZ - 1) / TZ;
+  __shared__ uint64_t E[RZ][RY], O[RZ][RY];
+  __shared__ uint32_t grown[FZ][FY];
+  const int gx = (nx + FX - 1) / FX, gy = (ny + FY - 1) / FY, gz = (nz + FZ - 1) / FZ;
   const int bxi = blockIdx.x % gx;
   const int byi = (blockIdx.x / gx) % gy;
   const int bzi = blockIdx.x / (gx * gy);
@@ -284,46 +314,94 @@ __global__ __launch_bounds__(kCellBlock) void k_flood_tile(const uint8_t* __rest
     if (threadIdx.x == 0) next[t] = 0;
     return;
   }
-  const int ox = bxi * TX - 1, oy = byi * TY - 1, oz = bzi * TZ - 1;
-  for (int k = threadIdx.x; k < HX * HY * HZ; k += kCellBlock) {
-    const int lx = k % HX, ly = (k / HX) % HY, lz = k / (HX * HY);
-    const int x = ox + lx, y = oy + ly, z = oz + lz;
-    uint8_t v = 2;
-    if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
-      const size_t cell = ((size_t)z * ny + y) * nx + x;
-      v = barrier[cell] ? 2 : (ext[cell] ? 1 : 0);
+  const int x0 = bxi * FX, y0 = byi * FY - 1, z0 = bzi * FZ - 1;   // y0, z0: halo row 0
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+  // load: a wave takes two rows at a time, lanes 0-31 the first row's cells,
+  // 32-63 the second's; ballots assemble the bits.  The halo cells x0 - 1
+  // and x0 + 32 of the same two rows come from lanes 0-3 of a second load.
+  // Eight row pairs are loaded before any is assembled, and the barrier and
+  // exterior bytes are loaded together, so the loads overlap (the loop is
+  // latency-bound otherwise).
+  constexpr int kPairs = RY * RZ / 2, kWaves = kCellBlock / 64, kBatch = 8;
+  // every lane loads from a clamped, in-grid address and the validity is
+  // applied afterwards: no branch around a load, so the loads of a batch go
+  // out before the first wait
+  auto clampi = [](int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); };
+  for (int p0 = wave; p0 < kPairs; p0 += kWaves * kBatch) {
+    uint8_t bar[kBatch], exv[kBatch], hbar[kBatch], hexv[kBatch];
+    bool ok[kBatch], hok[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      const int pair = p0 + kWaves * k;
+      const int row = 2 * pair + (lane >> 5);
+      const int y = y0 + row % RY, z = z0 + row / RY;
+      const int x = x0 + (lane & 31);
+      ok[k] = pair < kPairs && y >= 0 && y < ny && z >= 0 && z < nz && x < nx;
+      const size_t c = ((size_t)clampi(z, nz) * ny + clampi(y, ny)) * nx + clampi(x, nx);
+      bar[k] = barrier[c];
+      exv[k] = ext[c];
+      const int hrow = 2 * pair + ((lane >> 1) & 1);
+      const int hy = y0 + hrow % RY, hz = z0 + hrow / RY;
+      const int hx = (lane & 1) ? x0 + FX : x0 - 1;
+      hok[k] = lane < 4 && pair < kPairs && hy >= 0 && hy < ny && hz >= 0 && hz < nz &&
+               hx >= 0 && hx < nx;
+      const size_t hc = ((size_t)clampi(hz, nz) * ny + clampi(hy, ny)) * nx + clampi(hx, nx);
+      hbar[k] = barrier[hc];
+      hexv[k] = ext[hc];
     }
-    st[lz][ly][lx] = v;
-  }
-  __syncthreads();
-  // each thread owns the 4 cells (lx, ly, 1..4) of its column
-  const int lx = 1 + threadIdx.x % TX, ly = 1 + threadIdx.x / TX;
-  uint8_t owned_new = 0;  // bit z: cell became exterior in this launch
-  for (;;) {
-    int mine = 0;
-    for (int lz = 1; lz <= TZ; ++lz) {
-      if (st[lz][ly][lx] != 0) continue;
-      if (st[lz][ly][lx - 1] == 1 || st[lz][ly][lx + 1] == 1 || st[lz][ly - 1][lx] == 1 ||
-          st[lz][ly + 1][lx] == 1 || st[lz - 1][ly][lx] == 1 || st[lz + 1][ly][lx] == 1) {
-        st[lz][ly][lx] = 1;
-        owned_new |= (uint8_t)(1u << lz);
-        mine = 1;
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      const int pair = p0 + kWaves * k;
+      if (pair >= kPairs) break;   // uniform over the wave
+      const bool open = ok[k] && !bar[k], hopen = hok[k] && !hbar[k];
+      const uint64_t bo = __ballot(open), be = __ballot(open && exv[k]);
+      const uint64_t ho = __ballot(hopen), he = __ballot(hopen && hexv[k]);
+      if (lane == 0) {
+        for (int j = 0; j < 2; ++j) {
+          const int r = 2 * pair + j;
+          const uint64_t o = ((bo >> (32 * j)) & 0xffffffffull) << 1 |
+                             ((ho >> (2 * j)) & 1ull) | (((ho >> (2 * j + 1)) & 1ull) << 33);
+          const uint64_t e = ((be >> (32 * j)) & 0xffffffffull) << 1 |
+                             ((he >> (2 * j)) & 1ull) | (((he >> (2 * j + 1)) & 1ull) << 33);
+          O[r / RY][r % RY] = o;
+          E[r / RY][r % RY] = e;
+        }
       }
     }
-    // barrier + OR: every thread sees the same decision, and the next sweep
-    // starts only after every write of this one
+  }
+  __syncthreads();
+
+  const int ly = 1 + threadIdx.x % FY, lz = 1 + threadIdx.x / FY;
+  const uint64_t pro = O[lz][ly] & kInterior;
+  const uint64_t e0 = E[lz][ly];
+  uint64_t e = e0;
+  for (;;) {
+    const uint64_t nb = E[lz][ly - 1] | E[lz][ly + 1] | E[lz - 1][ly] | E[lz + 1][ly];
+    const uint64_t g = fill_x(e | (nb & pro), pro);
+    const int mine = g != e;
+    if (mine) {
+      E[lz][ly] = g;
+      e = g;
+    }
+    // barrier + OR: every thread sees the same decision, and the next
+    // iteration starts only after every write of this one
     if (!__syncthreads_or(mine)) break;
   }
-  if (owned_new) {
-    const int x = ox + lx, y = oy + ly;
-    for (int lz = 1; lz <= TZ; ++lz) {
-      if (!(owned_new & (1u << lz))) continue;
-      const int z = oz + lz;
-      ext[((size_t)z * ny + y) * nx + x] = 1;
+  const uint32_t gw = (uint32_t)((e & ~e0) >> 1);
+  grown[lz - 1][ly - 1] = gw;
+  const int any = __syncthreads_or(gw != 0);
+  if (any) {
+    // store the new exterior cells, two rows per wave instruction
+    for (int pair = wave; pair < FY * FZ / 2; pair += kCellBlock / 64) {
+      const int row = 2 * pair + (lane >> 5);
+      const int ry = row % FY, rz = row / FY;
+      const int x = x0 + (lane & 31);
+      if ((grown[rz][ry] >> (lane & 31)) & 1u)
+        ext[((size_t)(z0 + 1 + rz) * ny + (y0 + 1 + ry)) * nx + x] = 1;
     }
-    *changed = 1;
+    if (threadIdx.x == 0) *changed = 1;
   }
-  const int any = __syncthreads_or(owned_new != 0);
   if (threadIdx.x == 0) next[t] = any ? 1 : 0;
 }
 
@@ -450,7 +528,7 @@ extern "C" int cfd_voxelize(const float* pos, size_t nv, const uint32_t* idx, si
   int iters = 0;
   const unsigned cgrid = (unsigned)((N + kCellBlock - 1) / kCellBlock);
   const unsigned tgrid =
-      (unsigned)(((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * ((nz + TZ - 1) / TZ));
+      (unsigned)(((nx + FX - 1) / FX) * ((ny + FY - 1) / FY) * ((nz + FZ - 1) / FZ));
 
   VCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   for (auto& e : ev) VCK(hipEventCreate(&e));
