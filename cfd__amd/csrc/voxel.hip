@@ -183,6 +183,22 @@ __global__ __launch_bounds__(kRasterBlock) void k_raster(
     const uint32_t bx = (uint32_t)((int)hi[0] - x0 + 1), by = (uint32_t)((int)hi[1] - y0 + 1),
                    bz = (uint32_t)((int)hi[2] - z0 + 1);
     const uint32_t ncell = bx * by * bz;
+    // Without a signed distance the exact distance only decides d2 <= 0.75.
+    // A cell whose centre lies farther than sqrt(0.7575) from the triangle's
+    // plane is farther than that from the triangle, so its test is false: the
+    // 1 % margin covers the rounding of the exact distance, whose error grows
+    // like 1 / sin^2 of the triangle's sharpest angle, so only triangles with
+    // sin^2 > 1e-8 use the shortcut (error below 1e-7 cell^2 there); slivers
+    // and degenerate triangles always take the exact distance.
+    const double e1x = b[0] - a[0], e1y = b[1] - a[1], e1z = b[2] - a[2];
+    const double e2x = c[0] - a[0], e2y = c[1] - a[1], e2z = c[2] - a[2];
+    const double nx_ = e1y * e2z - e1z * e2y;
+    const double ny_ = e1z * e2x - e1x * e2z;
+    const double nz_ = e1x * e2y - e1y * e2x;
+    const double nn = nx_ * nx_ + ny_ * ny_ + nz_ * nz_;
+    const double ll = (e1x * e1x + e1y * e1y + e1z * e1z) * (e2x * e2x + e2y * e2y + e2z * e2z);
+    const bool shortcut = !dist2bits && nn > 1e-8 * ll;
+    const double far2 = 0.7575 * nn;
     for (uint32_t k = threadIdx.x; k < ncell; k += kRasterBlock) {
       const uint32_t r = k / bx;
       const int x = x0 + (int)(k - r * bx);
@@ -191,6 +207,10 @@ __global__ __launch_bounds__(kRasterBlock) void k_raster(
       const int z = z0 + (int)zr;
       const double p[3] = {x + 0.5, y + 0.5, z + 0.5};
       const uint32_t cell = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx + (uint32_t)x;
+      if (shortcut) {
+        const double sp = nx_ * (p[0] - a[0]) + ny_ * (p[1] - a[1]) + nz_ * (p[2] - a[2]);
+        if (sp * sp > far2) continue;
+      }
       const double d2 = dist2_point_tri(p, a, b, c);
       // the distance field is kept only when a signed distance is asked for
       if (dist2bits && d2 == d2) atomicMin(&dist2bits[cell], __float_as_uint((float)d2));
