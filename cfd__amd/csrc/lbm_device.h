@@ -79,31 +79,34 @@ struct StoreF16 {
 
 // The same float16 storage with stochastic rounding in the step's stores: the
 // float32 value is rounded up or down to a neighbouring float16 with
-// probability proportional to its distance (13 random bits added below the
-// float16 mantissa, then truncated), so the stored value is unbiased.  A
-// deterministic rounding loses every update smaller than half a float16 ulp
-// (~2e-4 relative): a slowly decaying flow then stops decaying.  Outside the
-// float16 normal range the store rounds to nearest.  Loads, init and
-// conversions are those of StoreF16.
+// probability proportional to its distance, so the stored value is unbiased.
+// A deterministic rounding loses every update smaller than half a float16 ulp
+// (~2e-4 relative): a slowly decaying flow then stops decaying.  The rounding
+// is the hardware's stochastic conversion (v_cvt_sr_f16_f32), which compares
+// the discarded bits with the upper bits of a 32-bit random word; it is
+// unbiased over the whole float16 range, subnormals included (measured on
+// this GPU against 65 536 draws per value).  Loads, init and conversions are
+// those of StoreF16.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
 struct StoreF16S : StoreF16 {
   static constexpr bool kStochastic = true;
-  static __device__ __forceinline__ T round_sr(float v, uint32_t bits13) {
-    uint32_t u = __float_as_uint(v * 4096.0f);
-    const uint32_t e = (u >> 23) & 0xffu;
-    if (e - 113u < 29u) u = (u + bits13) & 0xffffe000u;   // 2^-14 <= |x| < 2^15
-    return static_cast<T>(__uint_as_float(u));
+  static __device__ __forceinline__ T round_sr(float v, uint32_t rnd) {
+    const f16x2 zero = {};
+    return __builtin_amdgcn_cvt_sr_f16_f32(zero, v * 4096.0f, rnd, false)[0];
   }
   template <class P>
-  static __device__ __forceinline__ void st_sr(P p, float v, uint32_t bits13) {
-    *p = round_sr(v, bits13);
+  static __device__ __forceinline__ void st_sr(P p, float v, uint32_t rnd) {
+    *p = round_sr(v, rnd);
   }
-  static __device__ __forceinline__ Bits enc_sr(float v, uint32_t bits13) {
-    return __builtin_bit_cast(Bits, round_sr(v, bits13));
+  static __device__ __forceinline__ Bits enc_sr(float v, uint32_t rnd) {
+    return __builtin_bit_cast(Bits, round_sr(v, rnd));
   }
 };
 
-// 13 random bits for population i of a cell: one avalanche hash per cell and
-// step, then a cheap per-direction mix.
+// A 32-bit random word for population i of a cell: one avalanche hash per
+// cell and step, then a 24-bit multiply by an odd per-direction constant
+// (full rate, unlike a 32-bit multiply), folded with the hash.
 __device__ __forceinline__ uint32_t sr_hash(uint32_t cell, uint32_t seed) {
   uint32_t h = cell * 0x9E3779B1u ^ seed;
   h ^= h >> 16;
@@ -114,7 +117,8 @@ __device__ __forceinline__ uint32_t sr_hash(uint32_t cell, uint32_t seed) {
   return h;
 }
 __device__ __forceinline__ uint32_t sr_bits(uint32_t h, int i) {
-  return ((h ^ ((uint32_t)i * 0x632BE5ABu)) * 0x85EBCA6Bu) >> 19;
+  const uint32_t k = ((0x9E3779u * (2u * (uint32_t)i + 1u)) & 0xFFFFFFu) | 1u;
+  return __umul24(h, k) ^ h;
 }
 
 // Store population i of cell c: rounded to nearest, or stochastically for

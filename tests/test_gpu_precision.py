@@ -137,8 +137,14 @@ def test_sphere_tunnel_cd_and_mass_drift_vs_fp64(orc):
     print("SPHERE_160x80x64", json.dumps(res))
     assert res["fp32"]["bad"] == 0 and res["fp16"]["bad"] == 0 and res["fp16sr"]["bad"] == 0
     assert abs(res["fp16sr"]["cd"] - res["fp64"]["cd"]) < 0.005 * abs(res["fp64"]["cd"])
-    # unbiased rounding: the mass drift follows float64 (round to nearest loses ~5)
-    assert abs(res["fp16sr"]["mass1"] - res["fp64"]["mass1"]) < 0.05
+    # unbiased rounding: the mass follows float64 (round to nearest drifts by ~5).  Stochastic
+    # rounding is unbiased but not noise-free: a stored g of ~3e-3 has a float16 ulp of ~2e-6
+    # (after the 2^12 scale), a rounding error of std ~8e-7, so the 15.6 M stores of a step add
+    # ~3e-3 of mass noise and 1000 steps a random walk of std ~0.1.  Bound: 3 sigma, and a
+    # twentieth of the round-to-nearest drift.
+    d_sr = abs(res["fp16sr"]["mass1"] - res["fp64"]["mass1"])
+    d_rn = abs(res["fp16"]["mass1"] - res["fp64"]["mass1"])
+    assert d_sr < 0.3 and d_sr < d_rn / 20, (d_sr, d_rn)
     assert abs(res["fp32"]["cd"] - res["fp64"]["cd"]) < 1e-3 * abs(res["fp64"]["cd"])
     assert abs(res["fp16"]["cd"] - res["fp64"]["cd"]) < 0.05 * abs(res["fp64"]["cd"])
 
