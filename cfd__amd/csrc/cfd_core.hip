@@ -553,8 +553,9 @@ static int ensure_closure_buffers(cfd_solver* s) {
     HIPCK(hipMemsetAsync(s->wall8, 0, s->N, s->stream));
   }
   if (!s->fallbacks) {
-    HIPCK(hipMalloc(&s->fallbacks, sizeof(unsigned long long)));
-    HIPCK(hipMemsetAsync(s->fallbacks, 0, sizeof(unsigned long long), s->stream));
+    HIPCK(hipMalloc(&s->fallbacks, kFallbackSlots * sizeof(unsigned long long)));
+    HIPCK(hipMemsetAsync(s->fallbacks, 0, kFallbackSlots * sizeof(unsigned long long),
+                         s->stream));
   }
   return CFD_OK;
 }
@@ -603,11 +604,13 @@ int cfd_closure_fallbacks(cfd_solver* s, int64_t* count, int reset) {
   if (!s || !count) return fail(CFD_E_ARG, "cfd_closure_fallbacks: null argument");
   *count = 0;
   if (!s->fallbacks) return CFD_OK;
-  unsigned long long v = 0;
+  unsigned long long v[kFallbackSlots];
   HIPCK(hipStreamSynchronize(s->stream));
-  HIPCK(hipMemcpy(&v, s->fallbacks, sizeof v, hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(v, s->fallbacks, sizeof v, hipMemcpyDeviceToHost));
   if (reset) HIPCK(hipMemset(s->fallbacks, 0, sizeof v));
-  *count = (int64_t)v;
+  unsigned long long sum = 0;
+  for (int k = 0; k < kFallbackSlots; ++k) sum += v[k];
+  *count = (int64_t)sum;
   return CFD_OK;
 }
 

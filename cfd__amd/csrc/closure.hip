@@ -115,6 +115,8 @@ __device__ __forceinline__ void load_mlp(MlpPacked& dst, const MlpPacked* __rest
 
 // ------------------------------------------------------------------ closure step
 
+constexpr int kClosureBlock = 256;
+
 template <class S, bool MACROS, int LB>
 __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
                                                   const GLOBAL typename S::T* __restrict__ src,
@@ -189,10 +191,24 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   }
   const bool use = fluid && inside;
   if (!use) xb = v8bf{};
+  // the 19 pulled values wait in LDS while the MLP runs (fewer live registers
+  // at the matrix instructions); the compiler barrier keeps them from being
+  // forwarded in registers around it
+  __shared__ float s_gi[Q][kClosureBlock];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) s_gi[i][threadIdx.x] = gi[i];
+  asm volatile("" ::: "memory");
   const float nu = W.out_scale * softplus_fast(mlp_wave(xb, W));
+  asm volatile("" ::: "memory");
+#pragma unroll
+  for (int i = 0; i < Q; ++i) gi[i] = s_gi[i][threadIdx.x];
   const bool ok = use && isfinite(nu);
+  // fallback count: per wave, then per workgroup (below), then one atomic
+  // per workgroup into one of kFallbackSlots counters (a single counter took
+  // one atomic per wave on one address: ~30 us per step at 2.8 M cells)
+  __shared__ uint32_t s_fb[kClosureBlock / 64];
   const unsigned long long fb = __ballot(fluid && !ok);
-  if (fb && (threadIdx.x & 63) == 0) atomicAdd(fallbacks, (unsigned long long)__popcll(fb));
+  if ((threadIdx.x & 63) == 0) s_fb[threadIdx.x >> 6] = (uint32_t)__popcll(fb);
   const float tau = ok ? p.tau0 + 3.0f * nu : tau_s;
 
   float d[Q];
@@ -213,12 +229,17 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   }
   if (MACROS && live)
     macros[cell] = wall ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(M.ux, M.uy, M.uz, M.delta);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t n = 0;
+#pragma unroll
+    for (int w = 0; w < kClosureBlock / 64; ++w) n += s_fb[w];
+    if (n) atomicAdd(&fallbacks[blockIdx.x & (kFallbackSlots - 1)], (unsigned long long)n);
+  }
 }
 
-constexpr int kClosureBlock = 256;
-
 template <class S, bool MACROS, int LB>
-__global__ __launch_bounds__(kClosureBlock) void k_step_closure(
+__global__ __launch_bounds__(kClosureBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_step_closure(
     const typename S::T* __restrict__ src, typename S::T* __restrict__ dst,
     const uint8_t* __restrict__ tipo, const uint8_t* __restrict__ wall8,
     const MlpPacked* __restrict__ mlp, float4* __restrict__ macros, Geom g, StepParams p,

@@ -88,6 +88,9 @@ hipError_t launch_link_write(const uint8_t* tipo8, const Geom& g, const uint32_t
 // count) reductions.  `partials` must hold kReduceGrid * 3 doubles; result
 // lands in out[0..2].
 constexpr int kReduceGrid = 1024;
+// The learned closure counts fallback cells into this many counters (one
+// atomic per workgroup, spread over the slots); the count is their sum.
+constexpr int kFallbackSlots = 1024;
 hipError_t launch_forces(Store sto, const void* pop, const uint2* links, uint32_t nlinks,
                          const Geom& g, double* partials, double* out, hipStream_t st);
 hipError_t launch_mass(Store sto, const void* pop, const uint8_t* tipo8, const Geom& g,
