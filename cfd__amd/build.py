@@ -10,7 +10,6 @@ Run ``python -m cfd__amd.build`` or call :func:`build_all`.
 from __future__ import annotations
 
 import os
-import shutil
 import subprocess
 import sys
 from pathlib import Path

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from cfd__amd import tunel
-from cfd__amd.fechamento import Fechamento, gradiente, iniciar_mlp, treinar_mlp, N_PARAMS
+from cfd__amd.fechamento import Fechamento, gradiente, iniciar_mlp, treinar_mlp
 
 pytestmark = pytest.mark.gpu
 
