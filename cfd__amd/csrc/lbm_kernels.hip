@@ -257,9 +257,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 
           for (uint32_t cell = c0 + threadIdx.x; cell < c1; cell += BLOCK)
             step_cell<StoreF32, false, kBlockLog2, true>(cell, src, dst, tipo, nullptr, D.g,
                                                          D.p, f);
-          double part[3];
-          block_sum_into<BLOCK>(f, part);  // uniform branch: the whole workgroup
-          if (threadIdx.x == 0) {
+          // most tiles hold no body link: their workgroup skips the
+          // reduction and the atomics (adding exact zeros changes no sum)
+          const int links =
+              __syncthreads_or(f[0] != 0.0 || f[1] != 0.0 || f[2] != 0.0);
+          double part[3] = {0.0, 0.0, 0.0};
+          if (links) block_sum_into<BLOCK>(f, part);  // uniform: the whole workgroup
+          if (links && threadIdx.x == 0) {
             const int k = (int)((step + 1) / (uint32_t)L.sample_every) - 1;
             if (k < L.max_samples) {
               double* slot = L.samples + ((size_t)E.tunnel * L.max_samples + k) * 3;
