@@ -74,3 +74,27 @@ def test_no_device_raises_loudly():
     with pytest.raises(_lib.CFDError) as e:
         SolverMI355X.criar({"nx": 8, "ny": 8, "nz": 8})
     assert "device" in str(e.value).lower()
+
+
+def test_batch_and_closure_argument_errors():
+    """The batch and closure entry points reject bad arguments before any
+    device call, with a message naming the entry point."""
+    from cfd__amd import _lib
+    L = _lib.lib
+    out = C.c_void_p()
+    assert L.cfd_batch_create(None, 0, C.byref(out)) == -1
+    assert "cfd_batch_create" in L.cfd_last_error().decode()
+    assert L.cfd_batch_run(None, 10, 0, None, None) == -1
+    assert "cfd_batch_run" in L.cfd_last_error().decode()
+    assert L.cfd_batch_max_grid(256, None, None) == -1
+    assert "cfd_batch_max_grid" in L.cfd_last_error().decode()
+    X = np.zeros(8 * 64, np.float32)
+    y = np.zeros(64, np.float32)
+    params = _lib.Mlp()
+    cfg = _lib.TrainCfg(steps=10, batch=100, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=1)
+    rc = L.cfd_mlp_train(X.ctypes.data, y.ctypes.data, 64, C.byref(cfg), C.byref(params), None)
+    assert rc == -1 and "multiple of 64" in L.cfd_last_error().decode()
+    rc = L.cfd_mlp_infer(None, 0, None, None, None)
+    assert rc == -1 and "cfd_mlp_infer" in L.cfd_last_error().decode()
+    count = C.c_int64()
+    assert L.cfd_closure_fallbacks(None, C.byref(count), 0) == -1
