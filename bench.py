@@ -88,7 +88,7 @@ def build_case(precision: str):
     units = Units(lengthM=length_m, speedMs=speed, cellsPerLength=BODY_LENGTH, uLb=U_IN,
                   rollingRoad=False, lesCs=0.1)
     return dict(grid=g, tipo=tipo, esponja=esponja, area=area, units=units, vox=vox,
-                mesh=info, cont=cont)
+                mesh=info, cont=cont, pos=pos, idx=idx)
 
 
 def run_gpu(case, precision: str, steps: int, warmup: int):
@@ -168,11 +168,20 @@ def cpu_baseline(case, budget_s: float):
         if el >= budget_s or n >= 1000:
             break
     cells = nx * ny * nz
+    # the voxelizer beside it: the oracle's restatement, single-threaded like
+    # the reference's JavaScript, on the same 200k-triangle mesh (it always
+    # computes the signed distance, which the GPU run above skips)
+    t1 = time.perf_counter()
+    oracle.voxelize(case["pos"], case["idx"], GRID, 1.5, True)
+    vox_ms = (time.perf_counter() - t1) * 1e3
     return {"value": cells * n / el / 1e6, "unit": "MLUPS", "cores": oracle.threads(),
             "kind": "port",
             "sample": f"oracle float32 step (oracle/lbm_oracle.c, OpenMP) on the same "
                       f"320x160x128 vehicle cell types, {n} steps after 1 warm-up step, "
-                      f"{el:.1f} s"}
+                      f"{el:.1f} s",
+            "voxelizer": {"ms": round(vox_ms, 1), "cores": 1, "kind": "port",
+                          "sample": "oracle voxelizer (single thread, with the signed "
+                                    "distance) on the same 200k-triangle vehicle, 320x160x128"}}
 
 
 def sphere_case():
