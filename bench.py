@@ -79,6 +79,7 @@ def build_case(precision: str):
     nx, ny, nz = GRID
     g = {"nx": nx, "ny": ny, "nz": nz}
     pos, idx, info = meshes.vehicle(seed=7)
+    voxelizar(pos, idx, g, banda=1.5, selarFrestas=True, comSdf=False)   # device warm-up
     vox = voxelizar(pos, idx, g, banda=1.5, selarFrestas=True, comSdf=False)
     tipo, esponja, cont = tunel.montarTipos(g, vox["tipo"], esteira=False)
     area = tunel.areaFrontalCorpo(vox["tipo"], g)
