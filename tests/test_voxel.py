@@ -164,3 +164,32 @@ def test_gpu_voxelizer_repeatable_and_empty_mesh():
     assert np.array_equal(a["tipo"], b["tipo"]) and np.array_equal(a["sdf"], b["sdf"])
     e = voxelizar(np.zeros(0, np.float32), np.zeros(0, np.uint32), g)
     assert (e["tipo"] == 0).all() and (e["sdf"] == 2.5).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grid,seal,seed", [((45, 37, 21), True, 3), ((70, 33, 40), False, 4),
+                                            ((33, 17, 50), True, 5)])
+def test_gpu_voxelizer_equals_oracle_on_ragged_tiles(orc, grid, seal, seed):
+    """Grids whose sizes are not multiples of the flood tile (32 x 16 x 16)
+    in any axis, with several tiles per axis: a closed icosphere crossing
+    tile boundaries, a box with a slit and a random soup, against the oracle
+    (itself bit-exact to the reference's voxelizer), types, sdf and counts."""
+    from voxel_cases import box, random_soup
+
+    from cfd__amd.voxel import voxelizar
+    nx, ny, nz = grid
+    parts = [meshes.icosphere(3, min(grid) * 0.6, (nx * 0.45, ny * 0.5, nz * 0.5), seed),
+             box((nx * 0.7, ny * 0.2, nz * 0.2), (nx * 0.9, ny * 0.6, nz * 0.7), "x+", 0.4),
+             random_soup(40, grid, seed)]
+    pos = np.concatenate([p[0] for p in parts])
+    off, idx = 0, []
+    for p, i in parts:
+        idx.append(i + off)
+        off += p.size // 3
+    idx = np.concatenate(idx).astype(np.uint32)
+    r = voxelizar(pos, idx, {"nx": nx, "ny": ny, "nz": nz}, banda=1.5, selarFrestas=seal)
+    tipo, sdf, st = orc.voxelize(pos, idx, grid, 1.5, seal)
+    assert np.array_equal(r["tipo"], tipo)
+    assert np.array_equal(r["sdf"].view(np.uint32), sdf.view(np.uint32))
+    assert (r["stats"]["nCasca"], r["stats"]["nDentro"]) == (int(st[0]), int(st[1]))
+    assert (r["tipo"] == 2).sum() > 0   # a flooded interior exists
