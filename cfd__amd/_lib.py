@@ -107,6 +107,8 @@ def _load() -> C.CDLL:
                                          C.POINTER(C.c_double)]),
         "cfd_set_closure": (C.c_int, [P, C.POINTER(Closure)]),
         "cfd_set_wall_distance": (C.c_int, [P, P, C.c_size_t]),
+        "cfd_compute_wall_distance": (C.c_int, [P]),
+        "cfd_get_wall_distance": (C.c_int, [P, P, C.c_size_t]),
         "cfd_closure_fallbacks": (C.c_int, [P, C.POINTER(C.c_int64), C.c_int]),
         "cfd_mlp_train": (C.c_int, [P, P, C.c_size_t, C.POINTER(TrainCfg), C.POINTER(Mlp), P]),
         "cfd_mlp_grad": (C.c_int, [P, P, C.c_size_t, C.POINTER(Mlp), C.POINTER(Mlp),
@@ -140,7 +142,8 @@ EXPORTS = (
     "cfd_step", "cfd_step_timed", "cfd_set_tuning", "cfd_get_tuning", "cfd_forces", "cfd_update_macros", "cfd_read_macros",
     "cfd_mass", "cfd_synchronize", "cfd_bytes", "cfd_destroy", "cfd_voxelize",
     "cfd_copy_bandwidth", "cfd_last_error", "cfd_version",
-    "cfd_set_closure", "cfd_set_wall_distance", "cfd_closure_fallbacks", "cfd_mlp_train",
+    "cfd_set_closure", "cfd_set_wall_distance", "cfd_compute_wall_distance",
+    "cfd_get_wall_distance", "cfd_closure_fallbacks", "cfd_mlp_train",
     "cfd_mlp_grad", "cfd_mlp_infer",
     "cfd_batch_create", "cfd_batch_reset", "cfd_batch_run", "cfd_batch_results",
     "cfd_batch_get_populations", "cfd_batch_set_populations", "cfd_batch_max_grid",

@@ -547,6 +547,13 @@ int cfd_bytes(cfd_solver* s, int64_t* bytes_out) {
 
 void cfd_destroy(cfd_solver* s) { release(s); }
 
+// The closure's wall feature: distance in eighths of a cell, rounded, 0 for
+// walls and non-positive values, saturating at 255 (31.875 cells).
+static uint8_t quantize_wall(float d) {
+  const float v = d * 8.0f;
+  return !(v > 0.0f) ? 0 : (v >= 255.0f ? 255 : (uint8_t)(v + 0.5f));
+}
+
 static int ensure_closure_buffers(cfd_solver* s) {
   if (!s->wall8) {
     HIPCK(hipMalloc(&s->wall8, s->N));
@@ -591,11 +598,44 @@ int cfd_set_wall_distance(cfd_solver* s, const float* d, size_t n) {
   int rc = ensure_closure_buffers(s);
   if (rc) return rc;
   std::vector<uint8_t> q(n);
-  for (size_t k = 0; k < n; ++k) {
-    const float v = d[k] * 8.0f;
-    q[k] = !(v > 0.0f) ? 0 : (v >= 255.0f ? 255 : (uint8_t)(v + 0.5f));
-  }
+  for (size_t k = 0; k < n; ++k) q[k] = quantize_wall(d[k]);
   HIPCK(hipMemcpy(s->wall8, q.data(), n, hipMemcpyHostToDevice));
+  return CFD_OK;
+}
+
+int cfd_compute_wall_distance(cfd_solver* s) {
+  g_err.clear();
+  if (!s) return fail(CFD_E_ARG, "cfd_compute_wall_distance: null handle");
+  int rc = ensure_closure_buffers(s);
+  if (rc) return rc;
+  // d^2 -> wall8 for d^2 = 0..1023, and 255 at 1024 (every d >= 32)
+  constexpr size_t kTable = 1088;  // 1025 entries, padded to keep gx aligned
+  uint8_t table[kTable] = {};
+  for (uint32_t d2 = 0; d2 < 1024; ++d2) table[d2] = quantize_wall((float)std::sqrt((double)d2));
+  table[1024] = 255;
+  uint8_t* tmp = nullptr;
+  HIPCK(hipMalloc(&tmp, kTable + 3 * s->N + 2));
+  uint8_t* gx = tmp + kTable;
+  uint16_t* h2 = reinterpret_cast<uint16_t*>(gx + ((s->N + 1) & ~(size_t)1));
+  hipError_t e = hipMemcpyAsync(tmp, table, kTable, hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = launch_wall_distance(s->tipo8, s->wall8, gx, h2, tmp, s->geom, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(CFD_E_HIP, "cfd_compute_wall_distance: %s", hipGetErrorString(e));
+  return CFD_OK;
+}
+
+int cfd_get_wall_distance(cfd_solver* s, uint8_t* q, size_t n) {
+  g_err.clear();
+  if (!s || !q) return fail(CFD_E_ARG, "cfd_get_wall_distance: null argument");
+  if (n != s->N)
+    return fail(CFD_E_ARG, "cfd_get_wall_distance: %zu values, expected %zu", n, s->N);
+  if (!s->wall8) {
+    std::memset(q, 0, n);
+    return CFD_OK;
+  }
+  HIPCK(hipStreamSynchronize(s->stream));
+  HIPCK(hipMemcpy(q, s->wall8, n, hipMemcpyDeviceToHost));
   return CFD_OK;
 }
 

@@ -448,6 +448,101 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ params,
   }
 }
 
+// ------------------------------------------------------------------ wall distance
+//
+// Exact Euclidean distance transform of the wall cells (types 1, 4, 7), kept
+// only as far as the closure's wall8 feature can tell distances apart: wall8
+// is 8 d rounded and saturates at 255, so every d >= 32 (d^2 >= 1024) maps to
+// 255 and the search stops there.  Three separable passes over squared
+// distances (rows along x, then y, then z): the nearest wall of a cell within
+// 32 cells lies within 31 cells on each axis, so windows of +-31 cells (and a
+// row distance capped at 32) give the exact d^2 whenever d^2 < 1024 and a
+// value >= 1024 otherwise.  The last pass maps d^2 through a table built on
+// the host with the same quantization as cfd_set_wall_distance, so the field
+// equals the host quantization of the exact transform bit for bit.
+constexpr uint32_t kWdCap = 32, kWdCap2 = kWdCap * kWdCap;
+
+__device__ __forceinline__ bool wd_wall(const uint8_t* t8, size_t cell) {
+  return is_wall(t8[cell] & TYPE_MASK);
+}
+
+// Distance along x to the nearest wall cell of the row, capped at 32.  One
+// wave per 64-cell segment of a row: three ballots hold the wall bits of the
+// segment and of its two neighbours, and each lane reads its distance from
+// the leading / trailing zeros of the 64-bit windows that end / start at its
+// own cell.
+__global__ __launch_bounds__(256) void k_wd_rows(const uint8_t* __restrict__ t8,
+                                                 uint8_t* __restrict__ gx, uint32_t nx,
+                                                 uint32_t segs, uint32_t waves) {
+  const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wave >= waves) return;  // uniform across the wave
+  const int lane = (int)(threadIdx.x & 63);
+  const uint32_t row = wave / segs;
+  const int x = (int)((wave - row * segs) * 64) + lane;
+  const size_t base = (size_t)row * nx;
+  auto wall = [&](int xx) { return xx >= 0 && xx < (int)nx && wd_wall(t8, base + xx); };
+  const uint64_t prev = __ballot(wall(x - 64));
+  const uint64_t cur = __ballot(wall(x));
+  const uint64_t next = __ballot(wall(x + 64));
+  // bit 63 - k of left = cell x - k; bit k of right = cell x + k
+  const uint64_t left = (cur << (63 - lane)) | (lane == 63 ? 0ull : prev >> (lane + 1));
+  const uint64_t right = (cur >> lane) | (lane == 0 ? 0ull : next << (64 - lane));
+  uint32_t d = kWdCap;
+  if (left) d = min(d, (uint32_t)__clzll((long long)left));
+  if (right) d = min(d, (uint32_t)(__ffsll((long long)right) - 1));
+  if (x < (int)nx) gx[base + x] = (uint8_t)d;
+}
+
+// Squared distance over the x-y plane: min over |dy| <= 31 of dy^2 + gx^2,
+// capped at 1024.  The search stops once dy^2 alone reaches the best value.
+__global__ __launch_bounds__(256) void k_wd_cols(const uint8_t* __restrict__ gx,
+                                                 uint16_t* __restrict__ h2, Geom g) {
+  const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
+  if (cell >= g.N) return;
+  const uint32_t y = (cell / g.nx) % g.ny;
+  const uint32_t gc = gx[cell];
+  uint32_t best = min(gc * gc, kWdCap2);
+  for (uint32_t d = 1; d < kWdCap && d * d < best; ++d) {
+    if (y >= d) {
+      const uint32_t v = gx[cell - d * g.nx];
+      best = min(best, d * d + v * v);
+    }
+    if (y + d < g.ny) {
+      const uint32_t v = gx[cell + d * g.nx];
+      best = min(best, d * d + v * v);
+    }
+  }
+  h2[cell] = (uint16_t)best;
+}
+
+// Squared distance over the volume (min over |dz| <= 31 of dz^2 + h2), then
+// the quantized wall distance from the host-built table (index 1024 = any
+// d^2 >= 1024).
+__global__ __launch_bounds__(256) void k_wd_planes(const uint16_t* __restrict__ h2,
+                                                   const uint8_t* __restrict__ table,
+                                                   uint8_t* __restrict__ wall8, Geom g) {
+  const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
+  if (cell >= g.N) return;
+  const uint32_t z = cell / g.nxny;
+  uint32_t best = h2[cell];
+  for (uint32_t d = 1; d < kWdCap && d * d < best; ++d) {
+    if (z >= d) best = min(best, d * d + (uint32_t)h2[cell - d * g.nxny]);
+    if (z + d < g.nz) best = min(best, d * d + (uint32_t)h2[cell + d * g.nxny]);
+  }
+  wall8[cell] = table[min(best, kWdCap2)];
+}
+
+hipError_t launch_wall_distance(const uint8_t* tipo8, uint8_t* wall8, uint8_t* gx, uint16_t* h2,
+                                const uint8_t* table, const Geom& g, hipStream_t st) {
+  const uint32_t segs = (g.nx + 63) / 64, waves = g.ny * g.nz * segs;
+  hipLaunchKernelGGL(k_wd_rows, dim3((waves + 3) / 4), dim3(256), 0, st, tipo8, gx, g.nx, segs,
+                     waves);
+  const dim3 cells((g.N + 255) / 256);
+  hipLaunchKernelGGL(k_wd_cols, cells, dim3(256), 0, st, gx, h2, g);
+  hipLaunchKernelGGL(k_wd_planes, cells, dim3(256), 0, st, h2, table, wall8, g);
+  return hipGetLastError();
+}
+
 }  // namespace cfd
 
 // ------------------------------------------------------------------ host side

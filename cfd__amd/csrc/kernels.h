@@ -160,5 +160,9 @@ hipError_t launch_step_closure(Store sto, const void* src, void* dst, const uint
                                const uint8_t* wall8, const MlpPacked* mlp, float4* macros,
                                bool write_macros, const Geom& g, const StepParams& p,
                                unsigned long long* fallbacks, hipStream_t st);
+// Wall distance of the closure from the type field: gx (N bytes) and h2
+// (N uint16) are scratch, table[0..1024] maps d^2 (capped at 1024) to wall8.
+hipError_t launch_wall_distance(const uint8_t* tipo8, uint8_t* wall8, uint8_t* gx, uint16_t* h2,
+                                const uint8_t* table, const Geom& g, hipStream_t st);
 
 }  // namespace cfd

@@ -11,7 +11,10 @@ parameters.
     fech = Fechamento.novo(X_raw, y_nu)        # normalization from the data
     fech.treinar(X_raw, y_nu, passos=20000)    # Adam, lr 1e-3
     solver.definirFechamento(fech)
-    solver.definirDistanciaParede(distancia_parede(tipo, grade))
+    solver.calcularDistanciaParede()           # on the device, from the types
+
+distancia_parede (scipy, host) gives the same distances unquantized; it is
+what the training features use.
 """
 from __future__ import annotations
 

@@ -99,7 +99,7 @@ def rodar(caso: dict, passos: int, precisao: str = "fp32", fech: Fechamento | No
         s.rampa(1000 * fator, lambda f: s.configurar(inletU=[U * f, 0.0, 0.0], **cfg))
         s.inicializar()
         if fech is not None:
-            s.definirDistanciaParede(caso["parede"])
+            s.calcularDistanciaParede()
             s.definirFechamento(fech)
         media_desde = passos // 2 if media_desde is None else media_desde
         cd, cl = JanelaCd(10 ** 7), JanelaCd(10 ** 7)

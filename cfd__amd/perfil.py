@@ -14,7 +14,10 @@ work its algorithm needs, so that it can be set against the HBM peak:
 * forces: the body-link list (8 bytes per link cell: the cell and its mask
   of solid directions, built when the types are set) and one population per
   link;
-* mass: 19 populations and the type of every cell.
+* mass: 19 populations and the type of every cell;
+* wall distance of the learned closure (cfd_compute_wall_distance): per
+  cell the type and the row distance (rows), the row distance and the plane
+  distance (columns), the plane distance and the result (planes).
 
 usage: python -m cfd__amd.perfil [--repeticoes R] [--precisao fp32|fp16] [--copia]
 Prints one JSON object.  Run it under rocprofv3 --kernel-trace --stats for
@@ -98,7 +101,8 @@ def resumo(a) -> None:
 
     # algorithmic bytes per launch of each cell-array kernel (1-byte arrays)
     per_cell = {"k_dilate": 2, "k_seed": 2, "k_flood_tile": 3, "k_skin": 3, "k_final": 3,
-                "k_prep_types": 4 * 19 * 2 + 1}
+                "k_prep_types": 4 * 19 * 2 + 1, "k_wd_rows": 2, "k_wd_cols": 3,
+                "k_wd_planes": 3}
     lines = [f"# Kernels around the step, {N // (160 * 128)}x160x128 vehicle "
              f"({N / 1e6:.2f} M cells, {j['triangles']} triangles, {j['precisao']})", "",
              "Commands (run from the repository root on one MI355X):", "", "```", a.comando,
@@ -130,6 +134,11 @@ def resumo(a) -> None:
               f"call (launch, two-stage reduction, readback) {fo['us_per_call']:.1f} us",
               f"- mass: host-timed call {ma['us_per_call']:.1f} us, "
               f"{ma['bytes']:,} algorithmic bytes"]
+    if "parede" in j:
+        pa = j["parede"]
+        lines.append(f"- wall distance: host-timed call (scratch allocation, table upload, "
+                     f"three passes, free) {pa['us_per_call']:.1f} us; {pa['bytes']:,} "
+                     "algorithmic bytes over the three passes")
     if a.fetch and a.write:
         fe, wr = _counter(a.fetch, "FETCH_SIZE"), _counter(a.write, "WRITE_SIZE")
         kc = next(k for k in fe if "k_copy" in k)
@@ -213,6 +222,11 @@ def main():
         for _ in range(a.repeticoes):
             s.massa()
         t_massa = (time.perf_counter() - t0) / a.repeticoes
+        s.calcularDistanciaParede()
+        t0 = time.perf_counter()
+        for _ in range(a.repeticoes):
+            s.calcularDistanciaParede()
+        t_parede = (time.perf_counter() - t0) / a.repeticoes
     finally:
         s.destruir()
     bpp = 4 if a.precisao == "fp32" else 2
@@ -233,6 +247,7 @@ def main():
                    "bytes": bytes_forca, "GBps": bytes_forca / t_forca / 1e9},
         "massa": {"us_per_call": t_massa * 1e6, "bytes": bytes_massa,
                   "GBps": bytes_massa / t_massa / 1e9},
+        "parede": {"us_per_call": t_parede * 1e6, "bytes": 8 * N},
     }))
 
 

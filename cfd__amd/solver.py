@@ -228,6 +228,18 @@ class SolverMI355X:
         a = np.ascontiguousarray(d, dtype=np.float32).reshape(-1)
         check(lib.cfd_set_wall_distance(self._h, ptr(a), a.size))
 
+    def calcularDistanciaParede(self) -> None:
+        """Wall distance of the closure computed on the device from the types
+        last set (exact Euclidean distance to the nearest wall cell, in the
+        same 1/8-cell quantization as definirDistanciaParede)."""
+        check(lib.cfd_compute_wall_distance(self._h))
+
+    def lerDistanciaParede(self) -> np.ndarray:
+        """The quantized wall distance the closure reads (uint8, 1/8 cell)."""
+        q = np.empty(self.N, dtype=np.uint8)
+        check(lib.cfd_get_wall_distance(self._h, ptr(q), q.size))
+        return q
+
     def fallbacksFechamento(self, zerar: bool = True) -> int:
         """Fluid cell-steps that kept the Smagorinsky tau since the last reset."""
         n = C.c_int64()

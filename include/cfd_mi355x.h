@@ -205,6 +205,13 @@ typedef struct cfd_train_cfg {
 int cfd_set_closure(cfd_solver *s, const cfd_closure *c);
 /* Distance of every cell to the nearest wall cell, in cells (n = N). */
 int cfd_set_wall_distance(cfd_solver *s, const float *d, size_t n);
+/* The same field computed on the device from the types last set: exact
+ * Euclidean distance to the nearest wall cell (1, 4, 7) centre, quantized as
+ * cfd_set_wall_distance quantizes it.  No reference counterpart (the learned
+ * closure is an addition); it replaces a host distance transform. */
+int cfd_compute_wall_distance(cfd_solver *s);
+/* The quantized field the closure reads, in eighths of a cell (n = N). */
+int cfd_get_wall_distance(cfd_solver *s, uint8_t *q, size_t n);
 /* Fluid cell-steps that fell back to Smagorinsky since the last reset. */
 int cfd_closure_fallbacks(cfd_solver *s, int64_t *count, int reset);
 /* Train params in place on X (n x 8, normalized features) and y (n targets

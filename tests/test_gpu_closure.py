@@ -219,3 +219,65 @@ def test_closure_step_matches_oracle(orc):
     # and the closure changed the flow: not the Smagorinsky result
     smag, _ = _run(g, tipo, esp, g0, 10, None, None)
     assert np.abs(smag[:, ~wall] - got[:, ~wall]).max() > 1e-6
+
+
+def _edt_quantized(wall):
+    """Host checker: scipy's exact Euclidean transform, then the closure's
+    1/8-cell quantization (round half up, 0 in walls, saturating at 255)."""
+    from scipy.ndimage import distance_transform_edt
+    if not wall.any():
+        return np.full(wall.size, 255, np.uint8)
+    v = distance_transform_edt(~wall).astype(np.float32).reshape(-1) * np.float32(8)
+    q = np.floor(v + np.float32(0.5))
+    return np.where(v > 0, np.where(v >= 255, 255, q), 0).astype(np.uint8)
+
+
+def _wall_case(nome):
+    rng = np.random.default_rng(sum(nome.encode()))
+    if nome == "tunel":
+        g, tipo, _ = _tunnel()
+        return g, tipo
+    if nome == "veiculo":
+        from cfd__amd import estabilidade
+        tipo, _ = estabilidade.montar(7)
+        return dict(estabilidade.GRADE), tipo
+    dims = {"esparso_impar": (37, 23, 9), "linhas_130": (130, 40, 7),
+            "saturado": (150, 90, 30), "sem_paredes": (70, 9, 5), "so_paredes": (65, 3, 4)}[nome]
+    nx, ny, nz = dims
+    p = {"esparso_impar": 0.01, "linhas_130": 0.002, "saturado": 2e-5}.get(nome, 0.0)
+    tipo = np.zeros(nx * ny * nz, np.uint32)
+    r = rng.random(tipo.size)
+    tipo[r < p] = rng.choice([1, 4, 7], size=int((r < p).sum()))
+    tipo[(r >= p) & (r < 3 * p)] = rng.choice([2, 3, 5, 6, 9], size=int(((r >= p) & (r < 3 * p)).sum()))
+    if nome == "saturado":
+        tipo[:5] = 1
+    if nome == "so_paredes":
+        tipo[:] = 7
+    return {"nx": nx, "ny": ny, "nz": nz}, tipo
+
+
+@pytest.mark.parametrize("nome", ["tunel", "veiculo", "esparso_impar", "linhas_130", "saturado",
+                                  "sem_paredes", "so_paredes"])
+def test_device_wall_distance_equals_host_transform(nome):
+    """cfd_compute_wall_distance = the host transform through cfd_set_wall_distance,
+    byte for byte, on rows that are not multiples of 64, with far regions that
+    saturate, without walls and with walls only."""
+    from cfd__amd.solver import SolverMI355X
+    g, tipo = _wall_case(nome)
+    wall = np.isin(tipo, [1, 4, 7])
+    ref = _edt_quantized(wall.reshape(g["nz"], g["ny"], g["nx"]))
+    s = SolverMI355X.criar(g)
+    try:
+        s.definirTipos(tipo)
+        s.calcularDistanciaParede()
+        dev = s.lerDistanciaParede()
+        from cfd__amd.fechamento import distancia_parede
+        s.definirDistanciaParede(distancia_parede(tipo, g))
+        host = s.lerDistanciaParede()
+    finally:
+        s.destruir()
+    assert np.array_equal(host, ref)
+    bad = np.flatnonzero(dev != ref)
+    assert bad.size == 0, (bad[:10], dev[bad[:10]], ref[bad[:10]])
+    if nome == "saturado":
+        assert (ref == 255).sum() > 0.25 * ref.size and np.unique(ref).size > 200
