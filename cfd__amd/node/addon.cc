@@ -405,6 +405,7 @@ napi_value SetTuning(napi_env env, napi_callback_info info) {
   t.remap = (int32_t)prop_num(env, a[1], "remap", t.remap);
   t.layout = (int32_t)prop_num(env, a[1], "layout", t.layout);
   t.slice_pad = (int32_t)prop_num(env, a[1], "slicePad", t.slice_pad);
+  t.sweep = (int32_t)prop_num(env, a[1], "sweep", t.sweep);
   CFD_OK_OR_THROW(cfd_set_tuning(h->s, &t));
   return nullptr;
 }

@@ -189,18 +189,19 @@ class SolverMI355X:
     @property
     def ajuste(self) -> dict:
         """Launch tuning and population layout of the step kernel
-        (block, remap, layout 0 = SoA / 1 = 512-cell blocks, slice_pad)."""
+        (block, remap, layout 0 = SoA / 1 = 512-cell blocks, slice_pad,
+        sweep 0 = ascending / 1 = alternating cell order per step)."""
         t = _lib.Tuning()
         check(lib.cfd_get_tuning(self._h, C.byref(t)))
         return {"block": t.block, "remap": t.remap, "layout": t.layout,
-                "slice_pad": t.slice_pad}
+                "slice_pad": t.slice_pad, "sweep": t.sweep}
 
     @ajuste.setter
     def ajuste(self, v: dict) -> None:
         cur = self.ajuste
         cur.update(v)
         t = _lib.Tuning(int(cur["block"]), int(cur["remap"]), int(cur["layout"]),
-                        int(cur["slice_pad"]))
+                        int(cur["slice_pad"]), int(cur["sweep"]))
         check(lib.cfd_set_tuning(self._h, C.byref(t)))
 
     def sincronizar(self) -> None:

@@ -73,6 +73,9 @@ typedef struct cfd_tuning {
                            pop[cell/512][i][cell%512] (default; needs a
                            population set below 4 GiB, else SoA is used)        */
   int32_t slice_pad;    /* SoA only: extra elements between the 19 slices      */
+  int32_t sweep;        /* 0: every step walks the cells in ascending order;
+                           1: odd steps walk them in descending order, so a
+                           step first reads what the previous one wrote last */
 } cfd_tuning;
 
 typedef struct cfd_vox_stats {

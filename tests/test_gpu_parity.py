@@ -253,17 +253,22 @@ def test_force_link_list_follows_the_types(orc):
 
 def test_layouts_and_launch_shapes_are_bit_identical():
     """The population layout (SoA or 512-cell blocks) and the launch shape
-    (workgroup size, XCD chunking) change addressing only: the populations
-    after 25 steps agree bit for bit."""
+    (workgroup size, XCD chunking, ascending or alternating cell order)
+    change addressing only: the populations after 25 steps agree bit for
+    bit."""
     grid = (72, 40, 36)                 # 103 680 cells: not a multiple of 512
     tipo, esponja = _tunnel_types(grid, esteira=True)
     g0 = _random_state(grid, 11)
     cfg = dict(omegaPlus=1.85, lesCs=0.12, inletU=[0.06, 0.0, 0.0], beltU=[0.06, 0.0, 0.0],
                esponja=esponja, espessuraCL=5.0)
     out = {}
-    for t in ({"block": 256, "remap": 8, "layout": 1}, {"block": 512, "remap": 0, "layout": 1},
-              {"block": 256, "remap": 0, "layout": 1}, {"block": 512, "remap": 64, "layout": 1},
-              {"block": 256, "remap": 2, "layout": 0}, {"block": 512, "remap": 0, "layout": 0}):
+    for t in ({"block": 256, "remap": 8, "layout": 1, "sweep": 0},
+              {"block": 512, "remap": 0, "layout": 1, "sweep": 0},
+              {"block": 256, "remap": 0, "layout": 1, "sweep": 1},
+              {"block": 512, "remap": 64, "layout": 1, "sweep": 1},
+              {"block": 256, "remap": 8, "layout": 1, "sweep": 1},
+              {"block": 256, "remap": 2, "layout": 0, "sweep": 0},
+              {"block": 512, "remap": 0, "layout": 0, "sweep": 1}):
         s = _solver().criar({"nx": grid[0], "ny": grid[1], "nz": grid[2]})
         try:
             s.ajuste = t
@@ -281,3 +286,22 @@ def test_layouts_and_launch_shapes_are_bit_identical():
     for k, (p, m) in out.items():
         assert np.array_equal(p[:, ~wall], ref_p[:, ~wall]), k
         assert np.array_equal(m, ref_m), k
+
+
+def test_default_sweep_and_tuning_errors():
+    """Alternating order is the default for float16 storage and for float32
+    sets that fit the 256 MB last-level cache; bad tuning values raise."""
+    from cfd__amd._lib import CFDError
+    S = _solver()
+    for g, prec, want in (({"nx": 72, "ny": 40, "nz": 36}, "fp32", 1),
+                          ({"nx": 320, "ny": 160, "nz": 128}, "fp32", 0),
+                          ({"nx": 320, "ny": 160, "nz": 128}, "fp16", 1)):
+        s = S.criar(g, precisao=prec)
+        try:
+            assert s.ajuste["sweep"] == want, (g, prec)
+            for bad in ({"sweep": 2}, {"block": 128}, {"remap": 3}, {"layout": 2}):
+                with pytest.raises(CFDError):
+                    s.ajuste = bad
+            assert s.ajuste["sweep"] == want
+        finally:
+            s.destruir()
