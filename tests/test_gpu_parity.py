@@ -305,3 +305,48 @@ def test_default_sweep_and_tuning_errors():
             assert s.ajuste["sweep"] == want
         finally:
             s.destruir()
+
+
+def test_step_parity_soa_beyond_4gib(orc):
+    """A float32 population set above 4 GiB (480x360x330, 57 M cells, 4.33 GB)
+    falls back to the SoA layout with 64-bit slice offsets; two steps (A -> B
+    -> A) of a sphere tunnel agree with the oracle, and so do the macros and
+    the force."""
+    from cfd__amd._lib import CFDError
+    grid = (480, 360, 330)
+    nx, ny, nz = grid
+    N = nx * ny * nz
+    assert N * 19 * 4 > 1 << 32
+    tipo, esponja = _tunnel_types(grid, esteira=True)
+    cfg = dict(omegaPlus=1.8, lesCs=0.1, inletU=[0.05, 0.0, 0.0], beltU=[0.05, 0.0, 0.0],
+               esponja=esponja, espessuraCL=6.0)
+    rng = np.random.default_rng(404)
+    g0 = rng.standard_normal((19, N), dtype=np.float32)
+    g0 *= (2e-3 * lattice.W[:, None]).astype(np.float32)
+    s = _solver().criar({"nx": nx, "ny": ny, "nz": nz})
+    try:
+        assert s.ajuste["layout"] == 0
+        with pytest.raises(CFDError):
+            s.ajuste = {"layout": 1}
+        s.definirTipos(tipo)
+        s.configurar(**cfg)
+        s.frente = "A"
+        s.definirPopulacoes(g0)
+        s.passo(2)
+        got = s.lerPopulacoes()
+        m = s.lerMacros().reshape(-1, 4)
+        fg = s.forcaBruta()
+    finally:
+        s.destruir()
+    ref, mref = orc.run(grid, g0, tipo, orc.params_array(nx=nx, **cfg), 2)
+    del g0
+    ref = ref.reshape(19, N)
+    mask = _nonwall(tipo)
+    scale = float(np.abs(ref[:, mask]).max())
+    for i in range(19):
+        e = float(np.abs(got[i, mask] - ref[i, mask]).max()) / scale
+        assert e <= TOL, (i, e)
+    mr = mref.reshape(-1, 4)
+    assert _rel(m[:, :3].T, mr[:, :3].T) <= TOL
+    fo = orc.forces(grid, ref, tipo)
+    assert np.abs(fg - fo).max() <= 1e-5 * max(np.abs(fo).max(), 1e-12) + 1e-9, (fg, fo)
