@@ -141,6 +141,23 @@ class Fechamento:
         np.savez(caminho, params=self.params, shift=self.shift, scale=self.scale, lo=self.lo,
                  hi=self.hi, out_scale=np.float32(self.out_scale))
 
+    def salvarJson(self, caminho) -> None:
+        """The model as JSON for the Node class (carregarFechamento in
+        solver.js): every float32 written as the shortest decimal that reads
+        back to it."""
+        import json
+
+        def f32(a):
+            a = np.asarray(a, dtype=np.float32).reshape(-1)
+            if not np.isfinite(a).all():
+                raise ValueError("salvarJson: JSON has no infinities")
+            return [float(str(v)) for v in a]
+        d = {"params": f32(self.params), "shift": f32(self.shift), "scale": f32(self.scale),
+             "lo": f32(self.lo), "hi": f32(self.hi),
+             "out_scale": f32([self.out_scale])[0]}
+        with open(caminho, "w") as f:
+            json.dump(d, f)
+
     @classmethod
     def carregar(cls, caminho) -> "Fechamento":
         d = np.load(caminho, allow_pickle=False)

@@ -410,6 +410,67 @@ napi_value SetTuning(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// setClosure(h, Float32Array(1410) | null): the cfd_closure struct flat, in
+// its field order (the 1377 MLP parameters, in_shift, in_scale, in_lo,
+// in_hi, out_scale); null returns to Smagorinsky.
+napi_value SetClosure(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  Handle* h = handle(env, a[0]);
+  if (!h) return nullptr;
+  napi_valuetype t;
+  NAPI_OK(napi_typeof(env, a[1], &t));
+  if (t == napi_null || t == napi_undefined) {
+    CFD_OK_OR_THROW(cfd_set_closure(h->s, nullptr));
+    return nullptr;
+  }
+  float* d = nullptr;
+  size_t n = 0;
+  if (!view(env, a[1], napi_float32_array, &d, &n)) return nullptr;
+  static_assert(sizeof(cfd_closure) == 1410 * sizeof(float), "cfd_closure is 1410 floats");
+  if (n != 1410) {
+    napi_throw_range_error(env, nullptr, "cfdmi addon: a closure is 1410 float32 values");
+    return nullptr;
+  }
+  cfd_closure c;
+  std::memcpy(&c, d, sizeof c);
+  CFD_OK_OR_THROW(cfd_set_closure(h->s, &c));
+  return nullptr;
+}
+
+napi_value SetWallDistance(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  Handle* h = handle(env, a[0]);
+  if (!h) return nullptr;
+  float* d = nullptr;
+  size_t n = 0;
+  if (!view(env, a[1], napi_float32_array, &d, &n)) return nullptr;
+  CFD_OK_OR_THROW(cfd_set_wall_distance(h->s, d, n));
+  return nullptr;
+}
+
+napi_value ComputeWallDistance(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!args(env, info, 1, a)) return nullptr;
+  Handle* h = handle(env, a[0]);
+  if (!h) return nullptr;
+  CFD_OK_OR_THROW(cfd_compute_wall_distance(h->s));
+  return nullptr;
+}
+
+napi_value ClosureFallbacks(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!args(env, info, 2, a)) return nullptr;
+  Handle* h = handle(env, a[0]);
+  if (!h) return nullptr;
+  bool reset = false;
+  NAPI_OK(napi_get_value_bool(env, a[1], &reset));
+  int64_t n = 0;
+  CFD_OK_OR_THROW(cfd_closure_fallbacks(h->s, &n, reset ? 1 : 0));
+  return number(env, (double)n);
+}
+
 napi_value Version(napi_env env, napi_callback_info) {
   napi_value v;
   napi_create_string_utf8(env, cfd_version(), NAPI_AUTO_LENGTH, &v);
@@ -434,6 +495,9 @@ NAPI_MODULE_INIT() {
       {"readMacros", ReadMacros},   {"mass", Mass},
       {"synchronize", Synchronize}, {"bytes", Bytes},
       {"voxelize", Voxelize},       {"setTuning", SetTuning},
+      {"setClosure", SetClosure},   {"setWallDistance", SetWallDistance},
+      {"computeWallDistance", ComputeWallDistance},
+      {"closureFallbacks", ClosureFallbacks},
       {"version", Version},
   };
   for (const auto& f : fns) {

@@ -147,11 +147,53 @@ class SolverMI355X {
     return [v[0] - o[0], v[1] - o[1], v[2] - o[2]];
   }
 
+  /** Learned sub-grid closure in every following step (no reference
+   *  counterpart): modelo = {params (1377), shift, scale, lo, hi (8 each),
+   *  outScale}, as carregarFechamento returns it; null returns to
+   *  Smagorinsky.  Needs the wall distance (calcularDistanciaParede). */
+  definirFechamento(modelo) {
+    if (modelo === null || modelo === undefined) {
+      addon.setClosure(this._h, null);
+      return;
+    }
+    const c = new Float32Array(1410);
+    const partes = [['params', 1377], ['shift', 8], ['scale', 8], ['lo', 8], ['hi', 8]];
+    let k = 0;
+    partes.forEach(([nome, n]) => {
+      const v = modelo[nome];
+      if (!v || v.length !== n) throw new Error(`fechamento: ${nome} deve ter ${n} valores`);
+      c.set(v, k);
+      k += n;
+    });
+    c[k] = modelo.outScale;
+    addon.setClosure(this._h, c);
+  }
+
+  /** Wall distance of the closure, computed on the GPU from the types set. */
+  calcularDistanciaParede() { addon.computeWallDistance(this._h); }
+
+  /** Wall distance from the caller (Float32Array of N distances in cells). */
+  definirDistanciaParede(d) {
+    addon.setWallDistance(this._h, d instanceof Float32Array ? d : Float32Array.from(d));
+  }
+
+  /** Fluid cell-steps that kept the Smagorinsky tau since the last reset. */
+  fallbacksFechamento(zerar = true) { return addon.closureFallbacks(this._h, !!zerar); }
+
   /** Float64 sum of delta over non-wall cells and the non-finite cell count. */
   async massa() { return addon.mass(this._h); }
 
   /** Current front as Float32Array(19 N), layout pop[i*N + cell]. */
   async lerPopulacoes() { return addon.getPopulations(this._h, Q * this.N); }
+
+  /** Writes Float32Array(19 N) (same layout) into the current front. */
+  definirPopulacoes(pops) {
+    if (pops.length !== Q * this.N) {
+      throw new Error(`populacoes com ${pops.length} valores, esperado ${Q * this.N}`);
+    }
+    addon.setPopulations(this._h, pops instanceof Float32Array ? pops : Float32Array.from(pops));
+    this.passos = 0;
+  }
 
   sincronizar() { addon.synchronize(this._h); }
 
@@ -171,4 +213,14 @@ function voxelizar(positions, indices, g, opts) {
 
 const VOXEL = { FLUIDO: 0, CASCA: 1, DENTRO: 2 };
 
-module.exports = { SolverMI355X, voxelizar, VOXEL, addon };
+/** A closure model exported by cfd__amd.fechamento.Fechamento.salvarJson. */
+function carregarFechamento(caminho) {
+  const d = JSON.parse(require('fs').readFileSync(caminho, 'utf8'));
+  return {
+    params: Float32Array.from(d.params), shift: Float32Array.from(d.shift),
+    scale: Float32Array.from(d.scale), lo: Float32Array.from(d.lo), hi: Float32Array.from(d.hi),
+    outScale: d.out_scale,
+  };
+}
+
+module.exports = { SolverMI355X, voxelizar, carregarFechamento, VOXEL, addon };

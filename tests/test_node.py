@@ -19,7 +19,8 @@ pytestmark = pytest.mark.skipif(NODE is None or not ADDON.exists(),
 EXPORTS = ["create", "destroy", "configure", "setTypes", "initEquilibrium", "setPopulations",
            "getPopulations", "setFront", "getFront", "step", "stepTimed", "forces",
            "updateMacros", "readMacros", "mass", "synchronize", "bytes", "voxelize",
-           "setTuning", "version"]
+           "setTuning", "setClosure", "setWallDistance", "computeWallDistance",
+           "closureFallbacks", "version"]
 
 
 def _node(code: str) -> str:
@@ -135,3 +136,57 @@ def test_node_validation_page_matches_python(tmp_path):
     assert js["passos"] == py["passos"] and js["areaFrontal"] == py["areaFrontal"]
     assert js["cd"] == py["cd"], (js["cd"], py["cd"])
     assert abs(js["cd"] - 1.480) <= 0.0005
+
+
+@pytest.mark.gpu
+def test_node_closure_matches_python(tmp_path):
+    """The learned closure from Node (the shipped model through its JSON
+    export, wall distance on the GPU, float16 storage with stochastic
+    rounding) gives the Python run's populations bit for bit, and the
+    closure is in use: the result differs from the Smagorinsky step."""
+    from cfd__amd import lattice
+    from cfd__amd.fechamento import Fechamento
+    from cfd__amd.fechamento_caso import MODELO
+    from cfd__amd.solver import SolverMI355X
+    nx, ny, nz = 40, 24, 20
+    g = {"nx": nx, "ny": ny, "nz": nz}
+    corpo = tunel.corpoAnalitico(g, {"tipo": "esfera", "cx": 14, "cy": 12, "cz": 9, "raio": 4.5})
+    tipo, _, _ = tunel.montarTipos(g, corpo, esteira=False)
+    rng = np.random.default_rng(5)
+    Z, Y, X = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    u = [0.08 + 0.01 * np.sin(2 * np.pi * Y / ny), 0.01 * np.sin(2 * np.pi * Z / nz),
+         0.01 * np.cos(2 * np.pi * X / nx)]
+    g0 = lattice.equilibrium(np.zeros_like(X, dtype=np.float64), u).reshape(19, -1)
+    g0 = np.ascontiguousarray(g0 + 1e-4 * rng.standard_normal(g0.shape) * lattice.W[:, None],
+                              dtype=np.float32)
+    tipo.astype(np.uint32).tofile(tmp_path / "tipo.u32")
+    g0.tofile(tmp_path / "pops.f32")
+    subprocess.run([NODE, str(ROOT / "tests" / "node_closure.js"), str(tmp_path / "tipo.u32"),
+                    str(tmp_path / "pops.f32"), str(MODELO.with_suffix(".json")), str(nx),
+                    str(ny), str(nz), str(tmp_path / "js")], check=True, timeout=120)
+    js = json.loads((tmp_path / "js.json").read_text())
+    pjs = np.fromfile(tmp_path / "js.pops", dtype=np.float32).reshape(19, -1)
+
+    def run(fech):
+        s = SolverMI355X.criar(g, precisao="fp16sr")
+        try:
+            s.definirTipos(tipo)
+            s.configurar(omegaPlus=1.9, lesCs=0.1, inletU=[0.08, 0, 0], beltU=[0, 0, 0])
+            s.frente = "A"
+            s.definirPopulacoes(g0)
+            if fech is not None:
+                s.calcularDistanciaParede()
+                s.definirFechamento(fech)
+            s.passo(30)
+            return s.lerPopulacoes(), (s.fallbacksFechamento() if fech is not None else 0)
+        finally:
+            s.destruir()
+
+    py, fb = run(Fechamento.carregar(MODELO))
+    smag, _ = run(None)
+    wall = np.isin(tipo, [1, 4, 7])
+    assert np.array_equal(pjs[:, ~wall], py[:, ~wall])
+    assert js["fallbacks"] == fb and js["depois"] == 0
+    fluid = np.count_nonzero(tipo == 0)
+    assert fb < 0.5 * 30 * fluid, fb
+    assert not np.array_equal(smag[:, ~wall], py[:, ~wall])
