@@ -212,12 +212,14 @@ int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
     s->geom = make_geom(nx, ny, nz, kLayoutSoA, (uint32_t)s->tuning.slice_pad);
   }
   s->pop_bytes = s->geom.pop_elems * eb;
-  // Alternate the cell order per step where it measured faster: a population
-  // set that fits the 256 MB last-level cache (the next step then reads first
-  // what is still cached), or float16 storage, whose step is bound by
+  // Alternate the cell order per step where it measured faster: a set larger
+  // than half the 256 MB last-level cache (below that both sets stay cached
+  // in either order) that fits it whole, so that the next step first reads
+  // what is still cached, or any such float16 set, whose step is bound by
   // arithmetic and gained at every size measured.  A float32 set larger than
-  // the cache lost 1.5-2 % with it.
-  s->tuning.sweep = (s->store != Store::F32 || s->pop_bytes <= (256ull << 20)) ? 1 : 0;
+  // the cache lost 1.5-2 % with it, one of 62 MB about 1 %.
+  s->tuning.sweep = s->pop_bytes > (128ull << 20) &&
+                    (s->store != Store::F32 || s->pop_bytes <= (256ull << 20));
 
   auto bail = [&](int code) {
     release(s);

@@ -289,13 +289,16 @@ def test_layouts_and_launch_shapes_are_bit_identical():
 
 
 def test_default_sweep_and_tuning_errors():
-    """Alternating order is the default for float16 storage and for float32
-    sets that fit the 256 MB last-level cache; bad tuning values raise."""
+    """Alternating order is the default for sets above 128 MB in float16
+    and for float32 sets of 128-256 MB (one set fits the last-level cache,
+    two do not); bad tuning values raise."""
     from cfd__amd._lib import CFDError
     S = _solver()
-    for g, prec, want in (({"nx": 72, "ny": 40, "nz": 36}, "fp32", 1),
+    for g, prec, want in (({"nx": 72, "ny": 40, "nz": 36}, "fp32", 0),
+                          ({"nx": 240, "ny": 120, "nz": 96}, "fp32", 1),
                           ({"nx": 320, "ny": 160, "nz": 128}, "fp32", 0),
-                          ({"nx": 320, "ny": 160, "nz": 128}, "fp16", 1)):
+                          ({"nx": 320, "ny": 160, "nz": 128}, "fp16", 1),
+                          ({"nx": 160, "ny": 80, "nz": 64}, "fp16", 0)):
         s = S.criar(g, precisao=prec)
         try:
             assert s.ajuste["sweep"] == want, (g, prec)
