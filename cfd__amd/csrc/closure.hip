@@ -493,8 +493,16 @@ __global__ __launch_bounds__(256) void k_wd_rows(const uint8_t* __restrict__ t8,
   if (x < (int)nx) gx[base + x] = (uint8_t)d;
 }
 
+// The windowed searches below take offsets in groups of kWdGroup, all loads
+// of a group issued together (one memory round trip per group), and stop
+// once the group's first offset alone reaches the best value.  Candidates
+// past that point have d^2 >= best and cannot lower it, and out-of-range
+// neighbours contribute a sentinel whose square is >= 1024, so the result
+// is the plain min over the window.
+constexpr uint32_t kWdGroup = 8;
+
 // Squared distance over the x-y plane: min over |dy| <= 31 of dy^2 + gx^2,
-// capped at 1024.  The search stops once dy^2 alone reaches the best value.
+// capped at 1024.
 __global__ __launch_bounds__(256) void k_wd_cols(const uint8_t* __restrict__ gx,
                                                  uint16_t* __restrict__ h2, Geom g) {
   const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
@@ -502,14 +510,19 @@ __global__ __launch_bounds__(256) void k_wd_cols(const uint8_t* __restrict__ gx,
   const uint32_t y = (cell / g.nx) % g.ny;
   const uint32_t gc = gx[cell];
   uint32_t best = min(gc * gc, kWdCap2);
-  for (uint32_t d = 1; d < kWdCap && d * d < best; ++d) {
-    if (y >= d) {
-      const uint32_t v = gx[cell - d * g.nx];
-      best = min(best, d * d + v * v);
+  for (uint32_t d0 = 1; d0 < kWdCap && d0 * d0 < best; d0 += kWdGroup) {
+    uint32_t v[kWdGroup];
+#pragma unroll
+    for (uint32_t j = 0; j < kWdGroup; ++j) {
+      const uint32_t d = d0 + j;
+      const uint32_t lo = (d < kWdCap && y >= d) ? gx[cell - d * g.nx] : kWdCap;
+      const uint32_t hi = (d < kWdCap && y + d < g.ny) ? gx[cell + d * g.nx] : kWdCap;
+      v[j] = min(lo, hi);
     }
-    if (y + d < g.ny) {
-      const uint32_t v = gx[cell + d * g.nx];
-      best = min(best, d * d + v * v);
+#pragma unroll
+    for (uint32_t j = 0; j < kWdGroup; ++j) {
+      const uint32_t d = d0 + j;
+      best = min(best, d * d + v[j] * v[j]);
     }
   }
   h2[cell] = (uint16_t)best;
@@ -525,9 +538,17 @@ __global__ __launch_bounds__(256) void k_wd_planes(const uint16_t* __restrict__ 
   if (cell >= g.N) return;
   const uint32_t z = cell / g.nxny;
   uint32_t best = h2[cell];
-  for (uint32_t d = 1; d < kWdCap && d * d < best; ++d) {
-    if (z >= d) best = min(best, d * d + (uint32_t)h2[cell - d * g.nxny]);
-    if (z + d < g.nz) best = min(best, d * d + (uint32_t)h2[cell + d * g.nxny]);
+  for (uint32_t d0 = 1; d0 < kWdCap && d0 * d0 < best; d0 += kWdGroup) {
+    uint32_t v[kWdGroup];
+#pragma unroll
+    for (uint32_t j = 0; j < kWdGroup; ++j) {
+      const uint32_t d = d0 + j;
+      const uint32_t lo = (d < kWdCap && z >= d) ? (uint32_t)h2[cell - d * g.nxny] : kWdCap2;
+      const uint32_t hi = (d < kWdCap && z + d < g.nz) ? (uint32_t)h2[cell + d * g.nxny] : kWdCap2;
+      v[j] = min(lo, hi);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kWdGroup; ++j) best = min(best, (d0 + j) * (d0 + j) + v[j]);
   }
   wall8[cell] = table[min(best, kWdCap2)];
 }
