@@ -68,6 +68,7 @@ struct cfd_solver {
   // learned closure (off while mlp == nullptr)
   MlpPacked* mlp = nullptr;
   uint8_t* wall8 = nullptr;   // wall distance, 1/8 cell, saturating at 31.875
+  bool wall_given = false;    // wall8 holds the caller's distances, not the types' transform
   unsigned long long* fallbacks = nullptr;
 };
 
@@ -306,6 +307,8 @@ int build_links(cfd_solver* s) {
 }
 }  // namespace
 
+static int compute_wall(cfd_solver* s);
+
 int cfd_set_types(cfd_solver* s, const uint32_t* tipo, size_t n) {
   g_err.clear();
   if (!s || !tipo) return fail(CFD_E_ARG, "cfd_set_types: null argument");
@@ -319,7 +322,10 @@ int cfd_set_types(cfd_solver* s, const uint32_t* tipo, size_t n) {
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   (void)hipFree(d32);
   if (e != hipSuccess) return fail(CFD_E_HIP, "cfd_set_types: %s", hipGetErrorString(e));
-  return build_links(s);
+  const int rc = build_links(s);
+  if (rc) return rc;
+  // a wall distance derived from the types follows them
+  return s->wall8 && !s->wall_given ? compute_wall(s) : CFD_OK;
 }
 
 int cfd_init_equilibrium(cfd_solver* s) {
@@ -567,10 +573,33 @@ static uint8_t quantize_wall(float d) {
   return !(v > 0.0f) ? 0 : (v >= 255.0f ? 255 : (uint8_t)(v + 0.5f));
 }
 
+// wall8 = the quantized exact distance transform of the current types.
+static int compute_wall(cfd_solver* s) {
+  // d^2 -> wall8 for d^2 = 0..1023, and 255 at 1024 (every d >= 32)
+  constexpr size_t kTable = 1088;  // 1025 entries, padded to keep gx aligned
+  uint8_t table[kTable] = {};
+  for (uint32_t d2 = 0; d2 < 1024; ++d2) table[d2] = quantize_wall((float)std::sqrt((double)d2));
+  table[1024] = 255;
+  uint8_t* tmp = nullptr;
+  HIPCK(hipMalloc(&tmp, kTable + 3 * s->N + 2));
+  uint8_t* gx = tmp + kTable;
+  uint16_t* h2 = reinterpret_cast<uint16_t*>(gx + ((s->N + 1) & ~(size_t)1));
+  hipError_t e = hipMemcpyAsync(tmp, table, kTable, hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = launch_wall_distance(s->tipo8, s->wall8, gx, h2, tmp, s->geom, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(CFD_E_HIP, "wall distance: %s", hipGetErrorString(e));
+  s->wall_given = false;
+  return CFD_OK;
+}
+
+// The closure's buffers; a new wall-distance field starts as the transform
+// of the current types.
 static int ensure_closure_buffers(cfd_solver* s) {
   if (!s->wall8) {
     HIPCK(hipMalloc(&s->wall8, s->N));
-    HIPCK(hipMemsetAsync(s->wall8, 0, s->N, s->stream));
+    int rc = compute_wall(s);
+    if (rc) return rc;
   }
   if (!s->fallbacks) {
     HIPCK(hipMalloc(&s->fallbacks, kFallbackSlots * sizeof(unsigned long long)));
@@ -613,29 +642,15 @@ int cfd_set_wall_distance(cfd_solver* s, const float* d, size_t n) {
   std::vector<uint8_t> q(n);
   for (size_t k = 0; k < n; ++k) q[k] = quantize_wall(d[k]);
   HIPCK(hipMemcpy(s->wall8, q.data(), n, hipMemcpyHostToDevice));
+  s->wall_given = true;
   return CFD_OK;
 }
 
 int cfd_compute_wall_distance(cfd_solver* s) {
   g_err.clear();
   if (!s) return fail(CFD_E_ARG, "cfd_compute_wall_distance: null handle");
-  int rc = ensure_closure_buffers(s);
-  if (rc) return rc;
-  // d^2 -> wall8 for d^2 = 0..1023, and 255 at 1024 (every d >= 32)
-  constexpr size_t kTable = 1088;  // 1025 entries, padded to keep gx aligned
-  uint8_t table[kTable] = {};
-  for (uint32_t d2 = 0; d2 < 1024; ++d2) table[d2] = quantize_wall((float)std::sqrt((double)d2));
-  table[1024] = 255;
-  uint8_t* tmp = nullptr;
-  HIPCK(hipMalloc(&tmp, kTable + 3 * s->N + 2));
-  uint8_t* gx = tmp + kTable;
-  uint16_t* h2 = reinterpret_cast<uint16_t*>(gx + ((s->N + 1) & ~(size_t)1));
-  hipError_t e = hipMemcpyAsync(tmp, table, kTable, hipMemcpyHostToDevice, s->stream);
-  if (e == hipSuccess) e = launch_wall_distance(s->tipo8, s->wall8, gx, h2, tmp, s->geom, s->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  (void)hipFree(tmp);
-  if (e != hipSuccess) return fail(CFD_E_HIP, "cfd_compute_wall_distance: %s", hipGetErrorString(e));
-  return CFD_OK;
+  if (!s->wall8) return ensure_closure_buffers(s);  // computes the new field
+  return compute_wall(s);
 }
 
 int cfd_get_wall_distance(cfd_solver* s, uint8_t* q, size_t n) {

@@ -204,14 +204,17 @@ typedef struct cfd_train_cfg {
   uint64_t seed;                  /* minibatch draw                              */
 } cfd_train_cfg;
 
-/* Use the closure in every following step (NULL: back to Smagorinsky). */
+/* Use the closure in every following step (NULL: back to Smagorinsky).  Its
+ * wall-distance feature starts as cfd_compute_wall_distance's field. */
 int cfd_set_closure(cfd_solver *s, const cfd_closure *c);
-/* Distance of every cell to the nearest wall cell, in cells (n = N). */
+/* Distance of every cell to the nearest wall cell, in cells (n = N), from
+ * the caller; it stays until cfd_compute_wall_distance replaces it. */
 int cfd_set_wall_distance(cfd_solver *s, const float *d, size_t n);
-/* The same field computed on the device from the types last set: exact
- * Euclidean distance to the nearest wall cell (1, 4, 7) centre, quantized as
- * cfd_set_wall_distance quantizes it.  No reference counterpart (the learned
- * closure is an addition); it replaces a host distance transform. */
+/* The same field computed on the device from the types: exact Euclidean
+ * distance to the nearest wall cell (1, 4, 7) centre, quantized as
+ * cfd_set_wall_distance quantizes it, and recomputed by every later
+ * cfd_set_types.  No reference counterpart (the learned closure is an
+ * addition); it replaces a host distance transform. */
 int cfd_compute_wall_distance(cfd_solver *s);
 /* The quantized field the closure reads, in eighths of a cell (n = N). */
 int cfd_get_wall_distance(cfd_solver *s, uint8_t *q, size_t n);

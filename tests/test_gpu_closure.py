@@ -281,3 +281,39 @@ def test_device_wall_distance_equals_host_transform(nome):
     assert bad.size == 0, (bad[:10], dev[bad[:10]], ref[bad[:10]])
     if nome == "saturado":
         assert (ref == 255).sum() > 0.25 * ref.size and np.unique(ref).size > 200
+
+
+def test_wall_distance_follows_the_types_unless_given():
+    """The closure's wall distance starts as the device transform of the
+    types, follows later definirTipos calls, and keeps caller distances until
+    calcularDistanciaParede."""
+    from cfd__amd.solver import SolverMI355X
+    g, tipo_a, _ = _tunnel()
+    nx, ny, nz = g["nx"], g["ny"], g["nz"]
+    corpo = tunel.corpoAnalitico(g, {"tipo": "cilindro", "cx": nx * 0.6, "cy": ny * 0.5,
+                                     "cz": nz * 0.5, "raio": 3.0})
+    tipo_b, _, _ = tunel.montarTipos(g, corpo, esteira=False)
+
+    def ref(t):
+        return _edt_quantized(np.isin(t, [1, 4, 7]).reshape(nz, ny, nx))
+    assert not np.array_equal(ref(tipo_a), ref(tipo_b))
+    custom = np.random.default_rng(9).uniform(0, 40, g["nx"] * ny * nz).astype(np.float32)
+    q = np.floor(custom * np.float32(8) + np.float32(0.5))
+    q_custom = np.where(custom * np.float32(8) >= 255, 255, q).astype(np.uint8)
+    s = SolverMI355X.criar(g)
+    try:
+        s.definirTipos(tipo_a)
+        s.definirFechamento(_fech(1))
+        assert np.array_equal(s.lerDistanciaParede(), ref(tipo_a))
+        s.definirTipos(tipo_b)
+        assert np.array_equal(s.lerDistanciaParede(), ref(tipo_b))
+        s.definirDistanciaParede(custom)
+        assert np.array_equal(s.lerDistanciaParede(), q_custom)
+        s.definirTipos(tipo_a)
+        assert np.array_equal(s.lerDistanciaParede(), q_custom)
+        s.calcularDistanciaParede()
+        assert np.array_equal(s.lerDistanciaParede(), ref(tipo_a))
+        s.definirTipos(tipo_b)
+        assert np.array_equal(s.lerDistanciaParede(), ref(tipo_b))
+    finally:
+        s.destruir()
