@@ -147,18 +147,7 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   }
   const uint32_t ct = tb & TYPE_MASK;
   const bool wall = is_wall(ct);
-  if (!wall && (tb & NEAR_WALL)) {
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      const uint32_t t = tipo[sb[i] / (uint32_t)kSz] & TYPE_MASK;
-      if (is_wall(t)) {
-        const bool moving = (t == SOLIDO_MOVEL);
-        const float cu = dotc(i, moving ? p.belt[0] : 0.f, moving ? p.belt[1] : 0.f,
-                              moving ? p.belt[2] : 0.f);
-        gi[i] = in.ld(opp(i), own) + kwall(i) * cu;
-      }
-    }
-  }
+  if (!wall && (tb & NEAR_WALL)) bounce_back<false>(gi, sb, own, tipo, in, p, ct, nullptr);
 
   const Moments M = moments(gi);
   float pab[6];

@@ -335,6 +335,58 @@ __device__ __forceinline__ float dotc(int i, float vx, float vy, float vz) {
   return acc;
 }
 
+// Halfway bounce-back of a cell flagged NEAR_WALL (ir.js:166-189): for every
+// direction i whose upstream neighbour x - c_i is a wall (SOLIDO, SOLIDO_MOVEL,
+// PAREDE), g_i = src[opp i][x] + 2 w_i (c_i . u_w) / c_s^2 with u_w = beltU on
+// SOLIDO_MOVEL and 0 elsewhere.  sb[i] is the byte offset of x - c_i in units
+// of the population type, own the cell's own byte offset in the set.
+//
+// The 18 neighbour types are requested together and reduced to masks, then
+// every bounced direction is re-loaded under its mask bit before any is used:
+// two memory round trips however many links the cell has (a test-and-load per
+// direction waits on memory up to 36 times, which made the no-slip floor's
+// first fluid plane cost ~3 % of a 320x160x128 step).
+//
+// FORCE also accumulates, into f, the momentum exchange c_j 2 (g_j + w_j) of
+// every SOLIDO link of a FLUIDO cell (wgsl.js:358-379), j = opp(i), from the
+// re-loaded g_j before the wall term is added.
+template <bool FORCE, class S, int LB>
+__device__ __forceinline__ void bounce_back(float (&gi)[Q], const uint32_t (&sb)[Q],
+                                            uint32_t own, const GLOBAL uint8_t* __restrict__ tipo,
+                                            const PopView<S, LB>& in, const StepParams& p,
+                                            uint32_t ct, double* f) {
+  constexpr uint32_t kSz = (uint32_t)sizeof(typename S::T);
+  uint32_t tn[Q];
+#pragma unroll
+  for (int i = 1; i < Q; ++i) tn[i] = tipo[sb[i] / kSz];
+  uint32_t wall_m = 0u, moving_m = 0u, solid_m = 0u;
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    const uint32_t t = tn[i] & TYPE_MASK;
+    if (is_wall(t)) wall_m |= 1u << i;
+    if (t == SOLIDO_MOVEL) moving_m |= 1u << i;
+    if (t == SOLIDO) solid_m |= 1u << i;
+  }
+#pragma unroll
+  for (int i = 1; i < Q; ++i)
+    if ((wall_m >> i) & 1u) gi[i] = in.ld(opp(i), own);
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    if (!((wall_m >> i) & 1u)) continue;
+    const bool moving = (moving_m >> i) & 1u;
+    const float cu = dotc(i, moving ? p.belt[0] : 0.f, moving ? p.belt[1] : 0.f,
+                          moving ? p.belt[2] : 0.f);
+    if (FORCE && ((solid_m >> i) & 1u) && ct == FLUIDO) {
+      const int j = opp(i);
+      const double q = 2.0 * ((double)gi[i] + wd(j));
+      if (cx(j)) f[0] += cx(j) > 0 ? q : -q;
+      if (cy(j)) f[1] += cy(j) > 0 ? q : -q;
+      if (cz(j)) f[2] += cz(j) > 0 ? q : -q;
+    }
+    gi[i] = gi[i] + kwall(i) * cu;
+  }
+}
+
 // ------------------------------------------------------------------ cell physics
 
 // Moments of the pulled populations (ir.js:192-213), left-to-right sums as

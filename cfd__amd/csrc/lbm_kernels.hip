@@ -120,26 +120,7 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
     }
     return;
   }
-  if (tb & NEAR_WALL) {
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      const uint32_t t = tipo[sb[i] / (uint32_t)kSz] & TYPE_MASK;
-      if (is_wall(t)) {
-        const bool moving = (t == SOLIDO_MOVEL);
-        const float cu = dotc(i, moving ? p.belt[0] : 0.f, moving ? p.belt[1] : 0.f,
-                              moving ? p.belt[2] : 0.f);
-        const float back = in.ld(opp(i), own);
-        gi[i] = back + kwall(i) * cu;
-        if (FORCE && t == SOLIDO && ct == FLUIDO) {
-          const int j = opp(i);
-          const double q = 2.0 * ((double)back + wd(j));
-          if (cx(j)) f[0] += cx(j) > 0 ? q : -q;
-          if (cy(j)) f[1] += cy(j) > 0 ? q : -q;
-          if (cz(j)) f[2] += cz(j) > 0 ? q : -q;
-        }
-      }
-    }
-  }
+  if (tb & NEAR_WALL) bounce_back<FORCE>(gi, sb, own, tipo, in, p, ct, f);
 
   // 2. moments, then the branch chain ESPELHO_Y -> ESPELHO_Z -> ENTRADA ->
   //    SAIDA -> fluid (equilibrium, Smagorinsky, TRT, sponge).
