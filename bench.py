@@ -11,6 +11,9 @@ A "step" is one lattice update of the whole domain.  The timed region holds K
 steps in chunks of 100 (the smoothstep inflow ramp reconfigures between
 chunks, as the reference's passo() does) plus a momentum-exchange force sample
 after every chunk of the last 5000 steps.  value = MLUPS = cells * K / wall.
+The first 800 of the untimed warm-up steps choose the step's launch order
+(ascending, alternating or start-rotated; results are bit-identical in every
+order) by timing each on the live run, unless --ajuste fixes one.
 
 Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: this
 build runs on a single GPU.
@@ -45,7 +48,11 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default 20000 for the vehicle, 4000 for the sphere)")
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=1200,
+                    help="untimed steps; from 800 on, the first 800 choose the launch order")
+    ap.add_argument("--ajuste", default=None,
+                    help='fixed launch order as JSON, e.g. \'{"sweep": 0, "rotate": 9216}\' '
+                         "(skips the launch-order autotuner)")
     ap.add_argument("--precision", choices=("fp32", "fp16", "fp16sr"), default="fp32")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="budget of the CPU-baseline sample (0 disables it)")
@@ -92,7 +99,10 @@ def build_case(precision: str):
                 mesh=info, cont=cont, pos=pos, idx=idx)
 
 
-def run_gpu(case, precision: str, steps: int, warmup: int):
+AUTOTUNE_STEPS = 150            # per candidate order; 5 candidates of 150 + 10 steps
+
+
+def run_gpu(case, precision: str, steps: int, warmup: int, ajuste=None):
     from cfd__amd.forces import JanelaCd, coeficientes
     from cfd__amd.solver import SolverMI355X
 
@@ -112,8 +122,19 @@ def run_gpu(case, precision: str, steps: int, warmup: int):
     s.rampa(int(math.floor(g["nx"] / 0.05 + 0.5)), aplicar)
     s.inicializar()
 
-    # warm-up (untimed)
+    # warm-up (untimed), whose first 800 steps choose the launch order unless
+    # one is given (results are bit-identical in every order)
     done = 0
+    if ajuste is not None:
+        s.ajuste = ajuste
+        origem = "fixed (--ajuste)"
+    elif warmup >= 5 * (AUTOTUNE_STEPS + 10):
+        s.autoajustarPasso(passos=AUTOTUNE_STEPS)
+        done = 5 * (AUTOTUNE_STEPS + 10)
+        origem = "autotuned in the warm-up"
+    else:
+        origem = "default (warm-up too short to autotune)"
+    ordem = {k: s.ajuste[k] for k in ("block", "remap", "sweep", "rotate")}
     while done < warmup:
         n = min(CHUNK, warmup - done)
         s.passo(n)
@@ -138,7 +159,8 @@ def run_gpu(case, precision: str, steps: int, warmup: int):
     wall = time.perf_counter() - t0
     mass, nonfinite = s.massa()
     s.destruir()
-    return dict(wall=wall, kernel_ms=kernel_ms, cd=cd_win.media, cd_std=cd_win.desvio,
+    return dict(wall=wall, kernel_ms=kernel_ms, ordem=ordem, origem=origem,
+                cd=cd_win.media, cd_std=cd_win.desvio,
                 cl=cl_win.media, cl_std=cl_win.desvio, n_samples=cd_win.n, mass=mass,
                 nonfinite=nonfinite)
 
@@ -302,7 +324,8 @@ def main():
     case = build_case(precision)
     nx, ny, nz = GRID
     cells = nx * ny * nz
-    r = run_gpu(case, precision, args.steps, args.warmup)
+    r = run_gpu(case, precision, args.steps, args.warmup,
+                json.loads(args.ajuste) if args.ajuste else None)
     mlups = cells * args.steps / r["wall"] / 1e6
     kernel_avg_s = r["kernel_ms"] / 1e3 / args.steps
     bpc = BYTES_PER_CELL[precision]
@@ -332,6 +355,7 @@ def main():
             "re_requested": RE_REQUESTED, "floor": "PAREDE (no-slip)",
             "inlet_boundary_layer_cells": math.floor(nz * 0.12 + 0.5),
             "parallelism": "single GPU",
+            "launch_order": r["ordem"], "launch_order_source": r["origem"],
         },
         "hbm_frac_of_peak": round(achieved / HBM_PEAK_GBS, 4),
         "roofline": {

@@ -36,7 +36,8 @@ class Params(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block", C.c_int32), ("remap", C.c_int32), ("layout", C.c_int32),
-                ("slice_pad", C.c_int32), ("sweep", C.c_int32)]
+                ("slice_pad", C.c_int32), ("sweep", C.c_int32),
+                ("rotate", C.c_int32)]
 
 
 class TunnelSpec(C.Structure):

@@ -49,7 +49,7 @@ struct cfd_solver {
   Store store = Store::F32;
   size_t pop_bytes = 0;  // bytes of one population set
   Geom geom{};
-  Tuning tuning{256, 8, kLayoutBlocked, 0, 0};  // measured best on the 6.55 M-cell vehicle
+  Tuning tuning{256, 8, kLayoutBlocked, 0, 0, 0};  // measured best on the 6.55 M-cell vehicle
   StepParams params{};
   bool configured = false;
   int front = 0;  // 0 = A, 1 = B
@@ -397,6 +397,11 @@ int cfd_set_tuning(cfd_solver* s, const cfd_tuning* t) {
     return fail(CFD_E_ARG, "cfd_set_tuning: slice_pad %d outside [0, 2^20]", t->slice_pad);
   if (t->sweep != 0 && t->sweep != 1)
     return fail(CFD_E_ARG, "cfd_set_tuning: sweep %d must be 0 or 1", t->sweep);
+  if (t->rotate < 0)
+    return fail(CFD_E_ARG, "cfd_set_tuning: rotate %d must not be negative", t->rotate);
+  if (t->rotate > 0 && t->sweep != 0)
+    return fail(CFD_E_ARG, "cfd_set_tuning: rotate needs sweep 0 (the two orders exclude "
+                           "each other)");
   const size_t eb = elem_bytes(s);
   const Geom ng = make_geom(s->nx, s->ny, s->nz, t->layout, (uint32_t)t->slice_pad);
   if (t->layout == kLayoutBlocked && ng.pop_elems * eb >= (1ull << 32))
@@ -437,6 +442,7 @@ int cfd_set_tuning(cfd_solver* s, const cfd_tuning* t) {
   s->tuning.layout = t->layout;
   s->tuning.slice_pad = t->slice_pad;
   s->tuning.sweep = t->sweep;
+  s->tuning.rotate = t->rotate;
   return CFD_OK;
 }
 
@@ -448,6 +454,7 @@ int cfd_get_tuning(cfd_solver* s, cfd_tuning* t) {
   t->layout = s->tuning.layout;
   t->slice_pad = s->tuning.slice_pad;
   t->sweep = s->tuning.sweep;
+  t->rotate = s->tuning.rotate;
   return CFD_OK;
 }
 
@@ -477,7 +484,7 @@ static int enqueue_steps(cfd_solver* s, int n) {
     else
       HIPCK(launch_step(s->store, s->pop[f], s->pop[f ^ 1], s->tipo8, s->macros, k == n - 1,
                         s->geom, s->params, s->tuning, s->tuning.sweep && (s->sr_step & 1),
-                        s->stream));
+                        s->sr_step, s->stream));
     s->front = f ^ 1;
   }
   return CFD_OK;

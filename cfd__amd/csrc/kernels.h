@@ -60,6 +60,7 @@ struct Tuning {
   int layout;       // kLayoutSoA or kLayoutBlocked
   int slice_pad;    // SoA only: extra elements between population slices
   int sweep;        // 1: alternate the workgroup order (ascending / descending) per step
+  int rotate;       // > 0: step k starts k * rotate workgroups before step 0 (sweep 0 only)
 };
 
 // Population storage: float32 (152 B/cell/step) or scaled float16 (76 B),
@@ -71,7 +72,7 @@ hipError_t launch_prep_types(const uint32_t* tipo32, uint8_t* tipo8, const Geom&
 hipError_t launch_step(Store sto, const void* src, void* dst, const uint8_t* tipo8,
                        float4* macros, bool write_macros, const Geom& g,
                        const StepParams& p, const Tuning& tu, bool descending,
-                       hipStream_t st);
+                       uint64_t step, hipStream_t st);
 hipError_t launch_init(Store sto, void* dst, const uint8_t* tipo8, const Geom& g,
                        const StepParams& p, hipStream_t st);
 hipError_t launch_macros(Store sto, const void* src, float4* macros, const Geom& g,

@@ -147,11 +147,15 @@ __global__ __launch_bounds__(BLOCK) void k_step(const typename S::T* __restrict_
                                                 const uint8_t* __restrict__ tipo,
                                                 float4* __restrict__ macros, Geom g,
                                                 StepParams p, uint32_t chunk_log2,
-                                                uint32_t descending) {
+                                                uint32_t descending, uint32_t start) {
   // descending: the first workgroups dispatched take the last cells (a
   // bijection of the padded grid), so a step that follows an ascending one
-  // starts on the lines that step wrote last.
-  const uint32_t lb = logical_block_chunked(chunk_log2);
+  // starts on the lines that step wrote last.  start (< gridDim.x) rotates
+  // the order instead: the first workgroups take the blocks from start on, so
+  // a step that starts where the previous one's last blocks were reads those
+  // lines first.
+  uint32_t lb = logical_block_chunked(chunk_log2) + start;
+  if (lb >= gridDim.x) lb -= gridDim.x;
   const uint32_t cell = (descending ? gridDim.x - 1u - lb : lb) * BLOCK + threadIdx.x;
   if (cell >= g.N) return;
   step_cell<S, MACROS, LB, false>(cell, gptr(src), gptr(dst), gptr(tipo), macros, g, p,
@@ -622,7 +626,7 @@ hipError_t launch_prep_types(const uint32_t* t32, uint8_t* t8, const Geom& g, hi
 template <class S, bool M, int L>
 static void step_dispatch(const void* src, void* dst, const uint8_t* tipo8, float4* macros,
                           const Geom& g, const StepParams& p, const Tuning& tu,
-                          bool descending, hipStream_t st) {
+                          bool descending, uint64_t step, hipStream_t st) {
   auto s = static_cast<const typename S::T*>(src);
   auto d = static_cast<typename S::T*>(dst);
   const uint32_t nb = (g.N + tu.block - 1) / tu.block;
@@ -630,24 +634,27 @@ static void step_dispatch(const void* src, void* dst, const uint8_t* tipo8, floa
   while ((1 << lg) < tu.remap) ++lg;        // remap = chunk size, a power of two
   const uint32_t span = kXcds << lg;
   const dim3 grid(lg ? (nb + span - 1) / span * span : nb);
+  // step k starts k * rotate workgroups before step 0 (modulo the grid)
+  const uint64_t back = (uint64_t)(tu.rotate % grid.x) * (step % grid.x) % grid.x;
+  const uint32_t start = back ? grid.x - (uint32_t)back : 0u;
   if (tu.block == 512)
     hipLaunchKernelGGL((k_step<S, M, 512, L>), grid, dim3(512), 0, st, s, d, tipo8, macros, g,
-                       p, lg, (uint32_t)descending);
+                       p, lg, (uint32_t)descending, start);
   else
     hipLaunchKernelGGL((k_step<S, M, 256, L>), grid, dim3(256), 0, st, s, d, tipo8, macros, g,
-                       p, lg, (uint32_t)descending);
+                       p, lg, (uint32_t)descending, start);
 }
 
 hipError_t launch_step(Store sto, const void* src, void* dst, const uint8_t* tipo8,
                        float4* macros, bool write_macros, const Geom& g,
                        const StepParams& p, const Tuning& tu, bool descending,
-                       hipStream_t st) {
+                       uint64_t step, hipStream_t st) {
   if (tu.block != 256 && tu.block != 512) return hipErrorInvalidValue;
   CFD_DISPATCH(sto, g, {
     if (write_macros)
-      step_dispatch<S, true, L>(src, dst, tipo8, macros, g, p, tu, descending, st);
+      step_dispatch<S, true, L>(src, dst, tipo8, macros, g, p, tu, descending, step, st);
     else
-      step_dispatch<S, false, L>(src, dst, tipo8, macros, g, p, tu, descending, st);
+      step_dispatch<S, false, L>(src, dst, tipo8, macros, g, p, tu, descending, step, st);
   });
   return hipGetLastError();
 }

@@ -406,6 +406,7 @@ napi_value SetTuning(napi_env env, napi_callback_info info) {
   t.layout = (int32_t)prop_num(env, a[1], "layout", t.layout);
   t.slice_pad = (int32_t)prop_num(env, a[1], "slicePad", t.slice_pad);
   t.sweep = (int32_t)prop_num(env, a[1], "sweep", t.sweep);
+  t.rotate = (int32_t)prop_num(env, a[1], "rotate", t.rotate);
   CFD_OK_OR_THROW(cfd_set_tuning(h->s, &t));
   return nullptr;
 }

@@ -186,22 +186,46 @@ class SolverMI355X:
         self.passos += n
         return ms.value
 
+    def autoajustarPasso(self, passos: int = 150, fracoes=(0.02, 0.1, 0.36)) -> dict:
+        """Launch-order autotuner (no reference counterpart): times `passos`
+        steps in ascending and in alternating cell order and with the step start
+        rotated by each fraction of the launch grid, keeps the fastest and
+        returns the tuning.  Every order updates every cell from the same
+        previous state, so results are bit-identical whichever is chosen, and
+        the 5 * (passos + 10) steps taken here are ordinary steps of the run."""
+        base = self.ajuste
+        nb = -(-self.N // base["block"])
+        span = 8 * base["remap"] if base["remap"] > 1 else 1
+        grade = -(-nb // span) * span
+        cand = [{"sweep": 0, "rotate": 0}, {"sweep": 1, "rotate": 0}]
+        cand += [{"sweep": 0, "rotate": max(1, int(f * grade))} for f in fracoes]
+        best, best_ms = None, float("inf")
+        for c in cand:
+            self.ajuste = c
+            self.passo(10)          # leave the previous order's cache state behind
+            ms = self.passoCronometrado(passos)
+            if ms < best_ms:
+                best, best_ms = c, ms
+        self.ajuste = best
+        return self.ajuste
+
     @property
     def ajuste(self) -> dict:
         """Launch tuning and population layout of the step kernel
         (block, remap, layout 0 = SoA / 1 = 512-cell blocks, slice_pad,
-        sweep 0 = ascending / 1 = alternating cell order per step)."""
+        sweep 0 = ascending / 1 = alternating cell order per step, rotate R:
+        with sweep 0, step k starts k*R workgroups before step 0)."""
         t = _lib.Tuning()
         check(lib.cfd_get_tuning(self._h, C.byref(t)))
         return {"block": t.block, "remap": t.remap, "layout": t.layout,
-                "slice_pad": t.slice_pad, "sweep": t.sweep}
+                "slice_pad": t.slice_pad, "sweep": t.sweep, "rotate": t.rotate}
 
     @ajuste.setter
     def ajuste(self, v: dict) -> None:
         cur = self.ajuste
         cur.update(v)
         t = _lib.Tuning(int(cur["block"]), int(cur["remap"]), int(cur["layout"]),
-                        int(cur["slice_pad"]), int(cur["sweep"]))
+                        int(cur["slice_pad"]), int(cur["sweep"]), int(cur["rotate"]))
         check(lib.cfd_set_tuning(self._h, C.byref(t)))
 
     def sincronizar(self) -> None:

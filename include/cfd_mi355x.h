@@ -76,6 +76,10 @@ typedef struct cfd_tuning {
   int32_t sweep;        /* 0: every step walks the cells in ascending order;
                            1: odd steps walk them in descending order, so a
                            step first reads what the previous one wrote last */
+  int32_t rotate;       /* 0: off; R > 0 (sweep 0 only): step k starts k*R
+                           workgroups before step 0's start (modulo the grid),
+                           so its first R workgroups read what the previous
+                           step's last R wrote                                 */
 } cfd_tuning;
 
 typedef struct cfd_vox_stats {

@@ -253,9 +253,9 @@ def test_force_link_list_follows_the_types(orc):
 
 def test_layouts_and_launch_shapes_are_bit_identical():
     """The population layout (SoA or 512-cell blocks) and the launch shape
-    (workgroup size, XCD chunking, ascending or alternating cell order)
-    change addressing only: the populations after 25 steps agree bit for
-    bit."""
+    (workgroup size, XCD chunking, ascending, alternating or rotated cell
+    order) change addressing only: the populations after 25 steps agree bit
+    for bit."""
     grid = (72, 40, 36)                 # 103 680 cells: not a multiple of 512
     tipo, esponja = _tunnel_types(grid, esteira=True)
     g0 = _random_state(grid, 11)
@@ -268,7 +268,10 @@ def test_layouts_and_launch_shapes_are_bit_identical():
               {"block": 512, "remap": 64, "layout": 1, "sweep": 1},
               {"block": 256, "remap": 8, "layout": 1, "sweep": 1},
               {"block": 256, "remap": 2, "layout": 0, "sweep": 0},
-              {"block": 512, "remap": 0, "layout": 0, "sweep": 1}):
+              {"block": 512, "remap": 0, "layout": 0, "sweep": 1},
+              {"block": 256, "remap": 8, "layout": 1, "sweep": 0, "rotate": 37},
+              {"block": 512, "remap": 0, "layout": 0, "sweep": 0, "rotate": 150},
+              {"block": 256, "remap": 0, "layout": 1, "sweep": 0, "rotate": 1000}):
         s = _solver().criar({"nx": grid[0], "ny": grid[1], "nz": grid[2]})
         try:
             s.ajuste = t
@@ -302,7 +305,8 @@ def test_default_sweep_and_tuning_errors():
         s = S.criar(g, precisao=prec)
         try:
             assert s.ajuste["sweep"] == want, (g, prec)
-            for bad in ({"sweep": 2}, {"block": 128}, {"remap": 3}, {"layout": 2}):
+            for bad in ({"sweep": 2}, {"block": 128}, {"remap": 3}, {"layout": 2},
+                        {"sweep": 0, "rotate": -1}, {"sweep": 1, "rotate": 5}):
                 with pytest.raises(CFDError):
                     s.ajuste = bad
             assert s.ajuste["sweep"] == want
@@ -353,3 +357,35 @@ def test_step_parity_soa_beyond_4gib(orc):
     assert _rel(m[:, :3].T, mr[:, :3].T) <= TOL
     fo = orc.forces(grid, ref, tipo)
     assert np.abs(fg - fo).max() <= 1e-5 * max(np.abs(fo).max(), 1e-12) + 1e-9, (fg, fo)
+
+
+def test_launch_order_autotuner_keeps_the_trajectory():
+    """autoajustarPasso times the candidate launch orders on live steps and
+    keeps the fastest; since every order is bit-identical, the state after
+    the tuning steps plus 20 more equals plain stepping for as many steps."""
+    grid = (72, 40, 36)
+    tipo, esponja = _tunnel_types(grid, esteira=True)
+    g0 = _random_state(grid, 5)
+    cfg = dict(omegaPlus=1.8, lesCs=0.1, inletU=[0.05, 0.0, 0.0], beltU=[0.05, 0.0, 0.0],
+               esponja=esponja, espessuraCL=4.0)
+    out = []
+    for tune in (True, False):
+        s = _solver().criar({"nx": grid[0], "ny": grid[1], "nz": grid[2]})
+        try:
+            s.definirTipos(tipo)
+            s.configurar(**cfg)
+            s.frente = "A"
+            s.definirPopulacoes(g0)
+            if tune:
+                t = s.autoajustarPasso(passos=12)
+                assert t["sweep"] in (0, 1) and t["rotate"] >= 0
+                assert t["sweep"] == 0 or t["rotate"] == 0
+                assert s.passos == 5 * (12 + 10)
+                s.passo(20)
+            else:
+                s.passo(5 * (12 + 10) + 20)
+            out.append(s.lerPopulacoes())
+        finally:
+            s.destruir()
+    wall = np.isin(tipo, [1, 4, 7])
+    assert np.array_equal(out[0][:, ~wall], out[1][:, ~wall])
