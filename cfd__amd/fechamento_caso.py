@@ -324,6 +324,9 @@ def main(argv=None):
                     help="only the coarse float16 closure run (for rocprofv3)")
     ap.add_argument("--carregar", default=None,
                     help="judge a saved closure (.npz) instead of generating data and training")
+    ap.add_argument("--sementes", default=None,
+                    help="with --carregar: judge on these held-out vehicle seeds (e.g. 7,8,9) "
+                         "and summarize the Cd errors over them")
     a = ap.parse_args(argv)
     if a.perfil:
         # the coarse float16 closure run alone (kernel profiles), plus the copy
@@ -333,6 +336,32 @@ def main(argv=None):
         fech = Fechamento.carregar(a.carregar or MODELO)
         r = rodar(montar(AVALIACAO_SEED, 1), a.passos_grossos, "fp16", fech=fech)
         print(json.dumps({k: r[k] for k in ("cd", "mlups", "fallback_frac")}, default=float))
+        return
+    if a.carregar and a.sementes:
+        fech = Fechamento.carregar(a.carregar)
+        sementes = [int(x) for x in a.sementes.split(",")]
+        assert not set(sementes) & set(TREINO_SEEDS), "judging seeds must be held out"
+        res = {"por_semente": {}}
+        for sd in sementes:
+            r = avaliar(fech, seed=sd, passos_grossos=a.passos_grossos)
+            res["por_semente"][sd] = r["resumo"]
+            print(f"[sementes] {sd}: fine Cd {r['resumo']['cd_fino']:.4f}, errors "
+                  + ", ".join(f"{k} {v:+.3f}" for k, v in r["resumo"]["erro_cd"].items()),
+                  flush=True)
+        variantes = next(iter(res["por_semente"].values()))["erro_cd"].keys()
+        res["resumo"] = {
+            v: {"erro_abs_medio": float(np.mean([abs(r["erro_cd"][v])
+                                                 for r in res["por_semente"].values()])),
+                "erro_medio": float(np.mean([r["erro_cd"][v]
+                                             for r in res["por_semente"].values()])),
+                "dentro_3pct": int(sum(abs(r["erro_cd"][v]) <= 0.03
+                                       for r in res["por_semente"].values()))}
+            for v in variantes}
+        res["resumo"]["sementes"] = sementes
+        if a.saida:
+            with open(a.saida, "w") as fh:
+                json.dump(res, fh, indent=1, default=float)
+        print(json.dumps(res["resumo"], default=float))
         return
     if a.carregar:
         fech = Fechamento.carregar(a.carregar)
