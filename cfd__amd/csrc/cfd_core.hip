@@ -178,7 +178,8 @@ const char* cfd_last_error(void) { return g_err.c_str(); }
 // Lets the voxelizer translation unit report through cfd_last_error().
 void cfd__set_error(const char* msg) { g_err = msg ? msg : ""; }
 
-const char* cfd_version(void) { return "cfd_mi355x 0.1 gfx950"; }
+// 0.2: cfd_tuning gained rotate
+const char* cfd_version(void) { return "cfd_mi355x 0.2 gfx950"; }
 
 int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
   g_err.clear();
