@@ -411,6 +411,23 @@ napi_value SetTuning(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// getTuning(h) -> {block, remap, layout, slicePad, sweep, rotate}
+napi_value GetTuning(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!args(env, info, 1, a)) return nullptr;
+  Handle* h = handle(env, a[0]);
+  if (!h) return nullptr;
+  cfd_tuning t{};
+  CFD_OK_OR_THROW(cfd_get_tuning(h->s, &t));
+  napi_value out;
+  NAPI_OK(napi_create_object(env, &out));
+  const struct { const char* k; int32_t v; } f[] = {
+      {"block", t.block}, {"remap", t.remap}, {"layout", t.layout},
+      {"slicePad", t.slice_pad}, {"sweep", t.sweep}, {"rotate", t.rotate}};
+  for (const auto& e : f) napi_set_named_property(env, out, e.k, number(env, e.v));
+  return out;
+}
+
 // setClosure(h, Float32Array(1410) | null): the cfd_closure struct flat, in
 // its field order (the 1377 MLP parameters, in_shift, in_scale, in_lo,
 // in_hi, out_scale); null returns to Smagorinsky.
@@ -496,6 +513,7 @@ NAPI_MODULE_INIT() {
       {"readMacros", ReadMacros},   {"mass", Mass},
       {"synchronize", Synchronize}, {"bytes", Bytes},
       {"voxelize", Voxelize},       {"setTuning", SetTuning},
+      {"getTuning", GetTuning},
       {"setClosure", SetClosure},   {"setWallDistance", SetWallDistance},
       {"computeWallDistance", ComputeWallDistance},
       {"closureFallbacks", ClosureFallbacks},

@@ -116,15 +116,57 @@ class SolverMI355X {
     this.passos = 0;
   }
 
-  /** Enqueue n steps; the ramp callback runs first (solver.js:395-415). */
-  passo(n = 1) {
+  _avancarRampa(n) {
     const r = this._rampa;
     if (r && r.atual < r.passos) {
       r.atual += n;
       if (r.aplicar) r.aplicar(this.fatorRampa);
     }
+  }
+
+  /** Enqueue n steps; the ramp callback runs first (solver.js:395-415). */
+  passo(n = 1) {
+    this._avancarRampa(n);
     addon.step(this._h, n);
     this.passos += n;
+  }
+
+  /** passo(n), then wait; returns the device time of the n launches (ms). */
+  passoCronometrado(n) {
+    this._avancarRampa(n);
+    const ms = addon.stepTimed(this._h, n);
+    this.passos += n;
+    return ms;
+  }
+
+  /** Launch tuning of the step kernel (no reference counterpart): block,
+   *  remap, layout, slicePad, sweep, rotate.  Setting merges the given keys
+   *  into the current tuning; results are bit-identical under any tuning. */
+  get ajuste() { return addon.getTuning(this._h); }
+  set ajuste(t) { addon.setTuning(this._h, t); }
+
+  /** Launch-order autotuner (no reference counterpart): times `passos` live
+   *  steps in ascending and in alternating cell order and with the step start
+   *  rotated by each fraction of the launch grid, keeps the fastest and
+   *  returns the tuning.  The 5 * (passos + 10) steps are ordinary steps of
+   *  the run: every order gives the same bits. */
+  autoajustarPasso(passos = 150, fracoes = [0.02, 0.1, 0.36]) {
+    const base = this.ajuste;
+    const nb = Math.ceil(this.N / base.block);
+    const span = base.remap > 1 ? 8 * base.remap : 1;
+    const grade = Math.ceil(nb / span) * span;
+    const cand = [{ sweep: 0, rotate: 0 }, { sweep: 1, rotate: 0 }]
+      .concat(fracoes.map((f) => ({ sweep: 0, rotate: Math.max(1, Math.floor(f * grade)) })));
+    let melhor = null;
+    let melhorMs = Infinity;
+    for (const c of cand) {
+      this.ajuste = c;
+      this.passo(10);
+      const ms = this.passoCronometrado(passos);
+      if (ms < melhorMs) { melhor = c; melhorMs = ms; }
+    }
+    this.ajuste = melhor;
+    return this.ajuste;
   }
 
   /** Float32Array(N*4) of (ux, uy, uz, delta) (solver.js:427-442). */

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.skipif(NODE is None or not ADDON.exists(),
 EXPORTS = ["create", "destroy", "configure", "setTypes", "initEquilibrium", "setPopulations",
            "getPopulations", "setFront", "getFront", "step", "stepTimed", "forces",
            "updateMacros", "readMacros", "mass", "synchronize", "bytes", "voxelize",
-           "setTuning", "setClosure", "setWallDistance", "computeWallDistance",
+           "setTuning", "getTuning", "setClosure", "setWallDistance", "computeWallDistance",
            "closureFallbacks", "version"]
 
 
@@ -190,3 +190,46 @@ def test_node_closure_matches_python(tmp_path):
     fluid = np.count_nonzero(tipo == 0)
     assert fb < 0.5 * 30 * fluid, fb
     assert not np.array_equal(smag[:, ~wall], py[:, ~wall])
+
+
+@pytest.mark.gpu
+def test_node_launch_order_autotuner_keeps_the_trajectory(tmp_path):
+    """SolverMI355X.autoajustarPasso in Node: the state after the tuning
+    steps plus 20 more equals plain stepping for as many steps, bit for bit."""
+    grade = {"nx": 72, "ny": 40, "nz": 36}
+    tipo, esponja, _ = tunel.montarTipos(grade, None, esteira=True)
+    f = tmp_path / "tipo.u32"
+    tipo.astype(np.uint32).tofile(f)
+    code = f"""
+const fs = require('fs');
+const {{ SolverMI355X }} = require('./cfd__amd/node/solver.js');
+(async () => {{
+  const grade = {json.dumps(grade)};
+  const b = fs.readFileSync({json.dumps(str(f))});
+  const tipo = new Uint32Array(b.buffer, b.byteOffset, b.byteLength / 4);
+  const ctx = await SolverMI355X.criarDevice();
+  const out = [];
+  let escolha = null;
+  for (const tune of [true, false]) {{
+    const s = await SolverMI355X.criar(ctx, grade);
+    s.definirTipos(tipo);
+    s.configurar({{ omegaPlus: 1.8, lesCs: 0.1, inletU: [0.05, 0, 0], beltU: [0.05, 0, 0],
+                   esponja: {json.dumps(esponja)} }});
+    s.inicializar();
+    if (tune) {{ escolha = s.autoajustarPasso(12); s.passo(20); }}
+    else s.passo(5 * 22 + 20);
+    out.push({{ p: await s.lerPopulacoes(), passos: s.passos }});
+    s.destruir();
+  }}
+  const u32 = (x) => new Uint32Array(x.buffer, x.byteOffset, x.length);
+  const a = u32(out[0].p), c = u32(out[1].p);
+  let diff = 0;
+  for (let i = 0; i < a.length; i++) if (a[i] !== c[i]) diff++;
+  console.log(JSON.stringify({{ diff, n: a.length, passos: [out[0].passos, out[1].passos], escolha }}));
+}})().catch((e) => {{ console.error(e); process.exit(1); }});
+"""
+    r = json.loads(_node(code).strip().splitlines()[-1])
+    assert r["diff"] == 0 and r["n"] == 19 * 72 * 40 * 36, r
+    assert r["passos"] == [130, 130], r
+    e = r["escolha"]
+    assert e["sweep"] in (0, 1) and e["rotate"] >= 0 and (e["sweep"] == 0 or e["rotate"] == 0), e
