@@ -11,9 +11,10 @@ A "step" is one lattice update of the whole domain.  The timed region holds K
 steps in chunks of 100 (the smoothstep inflow ramp reconfigures between
 chunks, as the reference's passo() does) plus a momentum-exchange force sample
 after every chunk of the last 5000 steps.  value = MLUPS = cells * K / wall.
-The first 800 of the untimed warm-up steps choose the step's launch order
-(ascending, alternating or start-rotated; results are bit-identical in every
-order) by timing each on the live run, unless --ajuste fixes one.
+The first 1600 of the untimed warm-up steps choose the step's launch order and
+workgroup size (ascending, alternating or start-rotated, 256 or 512 threads;
+results are bit-identical in every order) by timing each on the live run,
+unless --ajuste fixes one.
 
 Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: this
 build runs on a single GPU.
@@ -48,8 +49,8 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default 20000 for the vehicle, 4000 for the sphere)")
-    ap.add_argument("--warmup", type=int, default=1200,
-                    help="untimed steps; from 800 on, the first 800 choose the launch order")
+    ap.add_argument("--warmup", type=int, default=2000,
+                    help="untimed steps; from 1600 on, the first 1600 choose the launch order")
     ap.add_argument("--ajuste", default=None,
                     help='fixed launch order as JSON, e.g. \'{"sweep": 0, "rotate": 9216}\' '
                          "(skips the launch-order autotuner)")
@@ -99,7 +100,8 @@ def build_case(precision: str):
                 mesh=info, cont=cont, pos=pos, idx=idx)
 
 
-AUTOTUNE_STEPS = 150            # per candidate order; 5 candidates of 150 + 10 steps
+AUTOTUNE_STEPS = 150            # per candidate; 10 candidates of 150 + 10 steps
+AUTOTUNE_TOTAL = 10 * (AUTOTUNE_STEPS + 10)
 
 
 def run_gpu(case, precision: str, steps: int, warmup: int, ajuste=None):
@@ -122,15 +124,15 @@ def run_gpu(case, precision: str, steps: int, warmup: int, ajuste=None):
     s.rampa(int(math.floor(g["nx"] / 0.05 + 0.5)), aplicar)
     s.inicializar()
 
-    # warm-up (untimed), whose first 800 steps choose the launch order unless
+    # warm-up (untimed), whose first 1600 steps choose the launch order unless
     # one is given (results are bit-identical in every order)
     done = 0
     if ajuste is not None:
         s.ajuste = ajuste
         origem = "fixed (--ajuste)"
-    elif warmup >= 5 * (AUTOTUNE_STEPS + 10):
+    elif warmup >= AUTOTUNE_TOTAL:
         s.autoajustarPasso(passos=AUTOTUNE_STEPS)
-        done = 5 * (AUTOTUNE_STEPS + 10)
+        done = AUTOTUNE_TOTAL
         origem = "autotuned in the warm-up"
     else:
         origem = "default (warm-up too short to autotune)"

@@ -145,18 +145,21 @@ class SolverMI355X {
   get ajuste() { return addon.getTuning(this._h); }
   set ajuste(t) { addon.setTuning(this._h, t); }
 
-  /** Launch-order autotuner (no reference counterpart): times `passos` live
-   *  steps in ascending and in alternating cell order and with the step start
-   *  rotated by each fraction of the launch grid, keeps the fastest and
-   *  returns the tuning.  The 5 * (passos + 10) steps are ordinary steps of
-   *  the run: every order gives the same bits. */
-  autoajustarPasso(passos = 150, fracoes = [0.02, 0.1, 0.36]) {
-    const base = this.ajuste;
-    const nb = Math.ceil(this.N / base.block);
-    const span = base.remap > 1 ? 8 * base.remap : 1;
-    const grade = Math.ceil(nb / span) * span;
-    const cand = [{ sweep: 0, rotate: 0 }, { sweep: 1, rotate: 0 }]
-      .concat(fracoes.map((f) => ({ sweep: 0, rotate: Math.max(1, Math.floor(f * grade)) })));
+  /** Launch-order autotuner (no reference counterpart): for each workgroup
+   *  size, times `passos` live steps in ascending and in alternating cell
+   *  order and with the step start rotated by each fraction of the launch
+   *  grid, keeps the fastest and returns the tuning.  The
+   *  blocos.length * (2 + fracoes.length) * (passos + 10) steps are ordinary
+   *  steps of the run: every order and shape gives the same bits. */
+  autoajustarPasso(passos = 150, fracoes = [0.02, 0.1, 0.36], blocos = [256, 512]) {
+    const remap = this.ajuste.remap;
+    const span = remap > 1 ? 8 * remap : 1;
+    let cand = [];
+    for (const block of blocos) {
+      const grade = Math.ceil(Math.ceil(this.N / block) / span) * span;
+      cand = cand.concat([{ block, sweep: 0, rotate: 0 }, { block, sweep: 1, rotate: 0 }],
+        fracoes.map((f) => ({ block, sweep: 0, rotate: Math.max(1, Math.floor(f * grade)) })));
+    }
     let melhor = null;
     let melhorMs = Infinity;
     for (const c of cand) {

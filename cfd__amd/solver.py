@@ -186,19 +186,26 @@ class SolverMI355X:
         self.passos += n
         return ms.value
 
-    def autoajustarPasso(self, passos: int = 150, fracoes=(0.02, 0.1, 0.36)) -> dict:
-        """Launch-order autotuner (no reference counterpart): times `passos`
-        steps in ascending and in alternating cell order and with the step start
-        rotated by each fraction of the launch grid, keeps the fastest and
-        returns the tuning.  Every order updates every cell from the same
-        previous state, so results are bit-identical whichever is chosen, and
-        the 5 * (passos + 10) steps taken here are ordinary steps of the run."""
-        base = self.ajuste
-        nb = -(-self.N // base["block"])
-        span = 8 * base["remap"] if base["remap"] > 1 else 1
-        grade = -(-nb // span) * span
-        cand = [{"sweep": 0, "rotate": 0}, {"sweep": 1, "rotate": 0}]
-        cand += [{"sweep": 0, "rotate": max(1, int(f * grade))} for f in fracoes]
+    def autoajustarPasso(self, passos: int = 150, fracoes=(0.02, 0.1, 0.36),
+                         blocos=(256, 512)) -> dict:
+        """Launch-order autotuner (no reference counterpart): for each
+        workgroup size, times `passos` steps in ascending and in alternating
+        cell order and with the step start rotated by each fraction of the
+        launch grid, keeps the fastest and returns the tuning.  Every order and
+        shape updates every cell from the same previous state, so results are
+        bit-identical whichever is chosen, and the
+        len(blocos) * (2 + len(fracoes)) * (passos + 10) steps taken here are
+        ordinary steps of the run."""
+        remap = self.ajuste["remap"]
+        cand = []
+        for block in blocos:
+            nb = -(-self.N // block)
+            span = 8 * remap if remap > 1 else 1
+            grade = -(-nb // span) * span
+            cand += [{"block": block, "sweep": 0, "rotate": 0},
+                     {"block": block, "sweep": 1, "rotate": 0}]
+            cand += [{"block": block, "sweep": 0, "rotate": max(1, int(f * grade))}
+                     for f in fracoes]
         best, best_ms = None, float("inf")
         for c in cand:
             self.ajuste = c

@@ -380,10 +380,11 @@ def test_launch_order_autotuner_keeps_the_trajectory():
                 t = s.autoajustarPasso(passos=12)
                 assert t["sweep"] in (0, 1) and t["rotate"] >= 0
                 assert t["sweep"] == 0 or t["rotate"] == 0
-                assert s.passos == 5 * (12 + 10)
+                assert s.passos == 10 * (12 + 10)
+                assert t["block"] in (256, 512)
                 s.passo(20)
             else:
-                s.passo(5 * (12 + 10) + 20)
+                s.passo(10 * (12 + 10) + 20)
             out.append(s.lerPopulacoes())
         finally:
             s.destruir()

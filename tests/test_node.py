@@ -217,7 +217,7 @@ const {{ SolverMI355X }} = require('./cfd__amd/node/solver.js');
                    esponja: {json.dumps(esponja)} }});
     s.inicializar();
     if (tune) {{ escolha = s.autoajustarPasso(12); s.passo(20); }}
-    else s.passo(5 * 22 + 20);
+    else s.passo(10 * 22 + 20);
     out.push({{ p: await s.lerPopulacoes(), passos: s.passos }});
     s.destruir();
   }}
@@ -230,6 +230,6 @@ const {{ SolverMI355X }} = require('./cfd__amd/node/solver.js');
 """
     r = json.loads(_node(code).strip().splitlines()[-1])
     assert r["diff"] == 0 and r["n"] == 19 * 72 * 40 * 36, r
-    assert r["passos"] == [130, 130], r
+    assert r["passos"] == [240, 240], r
     e = r["escolha"]
-    assert e["sweep"] in (0, 1) and e["rotate"] >= 0 and (e["sweep"] == 0 or e["rotate"] == 0), e
+    assert e["block"] in (256, 512) and e["sweep"] in (0, 1) and e["rotate"] >= 0 and (e["sweep"] == 0 or e["rotate"] == 0), e
