@@ -10,11 +10,19 @@ reference's Units does (units.js:157-164), so the run is at Re_lattice ~ 876.
 A "step" is one lattice update of the whole domain.  The timed region holds K
 steps in chunks of 100 (the smoothstep inflow ramp reconfigures between
 chunks, as the reference's passo() does) plus a momentum-exchange force sample
-after every chunk of the last 5000 steps.  value = MLUPS = cells * K / wall.
-The first 1600 of the untimed warm-up steps choose the step's launch order and
-workgroup size (ascending, alternating or start-rotated, 256 or 512 threads;
-results are bit-identical in every order) by timing each on the live run,
-unless --ajuste fixes one.
+after every chunk of the last 5000 steps; the line reports the sample window
+and flags a window that starts inside the 6400-step inflow ramp.  value =
+MLUPS = cells * K / wall.  The step kernel runs in the launch shape the
+library picks for this population set; a warm-up long enough for the
+launch-order autotuner (3000 steps) times the candidate shapes on live steps
+first and keeps the fastest (results are bit-identical in every shape), and
+--ajuste fixes one.
+
+After the timed region, and outside it: the precision check (the validation
+cases of the reference's validacao.html, sphere and cylinder at Re 100, plus
+the 160x80x64 sphere, each in float32, float16 and float16 with stochastic
+rounding: Cd and St error of the float16 forms against float32) and the CPU
+baseline (the oracle's float32 step on the host cores, and the voxelizer).
 
 Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: this
 build runs on a single GPU.
@@ -49,8 +57,8 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default 20000 for the vehicle, 4000 for the sphere)")
-    ap.add_argument("--warmup", type=int, default=2000,
-                    help="untimed steps; from 1600 on, the first 1600 choose the launch order")
+    ap.add_argument("--warmup", type=int, default=3000,
+                    help="untimed steps; from 3000 on, the first 3000 choose the launch shape")
     ap.add_argument("--ajuste", default=None,
                     help='fixed launch order as JSON, e.g. \'{"sweep": 0, "rotate": 9216}\' '
                          "(skips the launch-order autotuner)")
@@ -60,6 +68,8 @@ def parse_args():
     ap.add_argument("--workload", choices=("vehicle", "sphere"), default="vehicle",
                     help="vehicle: 320x160x128 (the headline); sphere: the 160x80x64 "
                          "icosphere case, 4000 steps")
+    ap.add_argument("--no-precision-check", dest="precision_check", action="store_false",
+                    help="skip the float16-vs-float32 Cd/St check after the timed region")
     ap.add_argument("--campos", default=None,
                     help="sphere workload: write the final macro field (VTK) here")
     args = ap.parse_args()
@@ -100,8 +110,8 @@ def build_case(precision: str):
                 mesh=info, cont=cont, pos=pos, idx=idx)
 
 
-AUTOTUNE_STEPS = 150            # per candidate; 10 candidates of 150 + 10 steps
-AUTOTUNE_TOTAL = 10 * (AUTOTUNE_STEPS + 10)
+RAMP_STEPS = int(math.floor(GRID[0] / 0.05 + 0.5))   # the app's ramp length (nx / 0.05)
+WINDOW = 5000                                          # Cd/Cl averaging window (last steps)
 
 
 def run_gpu(case, precision: str, steps: int, warmup: int, ajuste=None):
@@ -121,21 +131,23 @@ def run_gpu(case, precision: str, steps: int, warmup: int, ajuste=None):
     s.definirTipos(case["tipo"])
     aplicar(1.0)
     s.calibrarRepouso()
-    s.rampa(int(math.floor(g["nx"] / 0.05 + 0.5)), aplicar)
+    s.rampa(RAMP_STEPS, aplicar)
     s.inicializar()
 
-    # warm-up (untimed), whose first 1600 steps choose the launch order unless
-    # one is given (results are bit-identical in every order)
+    # warm-up (untimed); when it is long enough, its first steps choose the
+    # launch shape in calls of CHUNK steps, as the loop below takes them
+    # (results and the ramp are bit-identical in every shape)
     done = 0
+    tune_steps = SolverMI355X.passosAutoajuste(bloco=CHUNK)
     if ajuste is not None:
         s.ajuste = ajuste
         origem = "fixed (--ajuste)"
-    elif warmup >= AUTOTUNE_TOTAL:
-        s.autoajustarPasso(passos=AUTOTUNE_STEPS)
-        done = AUTOTUNE_TOTAL
+    elif warmup >= tune_steps:
+        s.autoajustarPasso(bloco=CHUNK)
+        done = tune_steps
         origem = "autotuned in the warm-up"
     else:
-        origem = "default (warm-up too short to autotune)"
+        origem = "library default for this population set (cfd_create)"
     ordem = {k: s.ajuste[k] for k in ("block", "remap", "sweep", "rotate")}
     while done < warmup:
         n = min(CHUNK, warmup - done)
@@ -144,7 +156,8 @@ def run_gpu(case, precision: str, steps: int, warmup: int, ajuste=None):
     s.sincronizar()
 
     cd_win, cl_win = JanelaCd(10 ** 6), JanelaCd(10 ** 6)
-    sample_from = max(0, steps - 5000)
+    sample_from = max(0, steps - WINDOW)
+    first_sample = None
     kernel_ms = 0.0
     t0 = time.perf_counter()
     done = 0
@@ -157,14 +170,65 @@ def run_gpu(case, precision: str, steps: int, warmup: int, ajuste=None):
             co = coeficientes(f, units, case["area"], g, corrigirBloqueio=True, theta=2.5)
             cd_win.adicionar(co["cd"])
             cl_win.adicionar(co["cl"])
+            if first_sample is None:
+                first_sample = warmup + done
     s.sincronizar()
     wall = time.perf_counter() - t0
     mass, nonfinite = s.massa()
     s.destruir()
     return dict(wall=wall, kernel_ms=kernel_ms, ordem=ordem, origem=origem,
                 cd=cd_win.media, cd_std=cd_win.desvio,
-                cl=cl_win.media, cl_std=cl_win.desvio, n_samples=cd_win.n, mass=mass,
+                cl=cl_win.media, cl_std=cl_win.desvio, n_samples=cd_win.n,
+                first_sample=first_sample, last_step=warmup + steps, mass=mass,
                 nonfinite=nonfinite)
+
+
+def _finite(x):
+    return x if x is not None and math.isfinite(x) else None
+
+
+def precision_check():
+    """Cd (and St) of the float16 storage forms against plain float32 on the
+    reference's validation cases (validacao.js:151-205: sphere Re 100 at
+    240x120x96, cylinder Re 100 at 480x8x320 with its shedding) and on the
+    160x80x64 sphere workload; every run on the GPU, after the timed region."""
+    from cfd__amd.caso import rodar_caso
+    from cfd__amd.validacao import CASOS
+
+    out = {}
+    for cid in ("esfera-re100", "cilindro-re100"):
+        caso = next(c for c in CASOS if c["id"] == cid)
+        runs = {}
+        for prec in ("fp32", "fp16", "fp16sr"):
+            r = rodar_caso(caso, precisao=prec)
+            runs[prec] = {"cd": r["cd"], "st": r.get("st"), "nonfinite": r["naoFinitos"],
+                          "steps": r["passos"]}
+        out[cid] = runs
+    c = sphere_case()
+    runs = {}
+    for prec in ("fp32", "fp16", "fp16sr"):
+        cd, mass, bad, _, _, _ = run_sphere(c, prec, 4000, 0)
+        runs[prec] = {"cd": cd, "mass": mass, "nonfinite": bad, "steps": 4000}
+    out["esfera-d16-160x80x64"] = runs
+    for runs in out.values():
+        ref = runs["fp32"]
+        for prec in ("fp16", "fp16sr"):
+            r = runs[prec]
+            r["cd_err_vs_fp32"] = _finite((r["cd"] - ref["cd"]) / ref["cd"])
+            if r.get("st") is not None and ref.get("st"):
+                r["st_err_vs_fp32"] = _finite((r["st"] - ref["st"]) / ref["st"])
+    return out
+
+
+def reference_voxelizer_ms():
+    """The reference's own voxelizar() (src/geom/voxel.js) on this 200k-triangle
+    vehicle, timed under Node when the golden fixtures were written."""
+    f = ROOT / "tests" / "golden" / "voxel_golden.json"
+    try:
+        case = json.loads(f.read_text())["cases"]["vehicle_200k_320x160x128"]
+        return case.get("ms_reference")
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(case, budget_s: float):
@@ -206,7 +270,19 @@ def cpu_baseline(case, budget_s: float):
                       f"{el:.1f} s",
             "voxelizer": {"ms": round(vox_ms, 1), "cores": 1, "kind": "port",
                           "sample": "oracle voxelizer (single thread, with the signed "
-                                    "distance) on the same 200k-triangle vehicle, 320x160x128"}}
+                                    "distance) on the same 200k-triangle vehicle, 320x160x128",
+                          "reference": {
+                              "ms": _r1(reference_voxelizer_ms()), "cores": 1,
+                              "kind": "reference",
+                              "sample": "the reference's own voxelizar() (src/geom/voxel.js, "
+                                        "Node 12, one thread) on this 200k-triangle vehicle "
+                                        "at 320x160x128, timed in the build container when "
+                                        "tests/golden/voxel_golden.json was written (the "
+                                        "reference does not travel to the GPU box)"}}}
+
+
+def _r1(x):
+    return round(x, 1) if x is not None else None
 
 
 def sphere_case():
@@ -233,42 +309,53 @@ def sphere_case():
                 voxel_ms=vox_ms)
 
 
-def main_sphere(args):
+def run_sphere(c, precision: str, steps: int, warmup: int, campos=None):
+    """The sphere workload on the GPU: Cd over the last 1000 steps (a sample
+    every 100), final mass and non-finite count, wall and kernel time."""
     from cfd__amd.forces import JanelaCd, coeficientes
     from cfd__amd.solver import SolverMI355X
 
-    c = sphere_case()
     g = c["grid"]
-    cells = g["nx"] * g["ny"] * g["nz"]
     cfg = dict(omegaPlus=c["omega"], lesCs=0.1, inletU=[c["u"], 0, 0], esponja=c["esponja"])
-    s = SolverMI355X.criar(g, precisao=args.precision)
+    s = SolverMI355X.criar(g, precisao=precision)
     try:
         s.definirTipos(c["tipo"])
         s.configurar(**cfg)
         s.calibrarRepouso()
         s.inicializar()
-        s.passo(max(args.warmup, 0))
-        s.sincronizar()
-        s.inicializar()          # the timed run starts from the initial state again
+        if warmup > 0:
+            s.passo(warmup)
+            s.sincronizar()
+            s.inicializar()          # the timed run starts from the initial state again
         jan = JanelaCd(10 ** 6)
         kernel_ms = 0.0
         t0 = time.perf_counter()
         done = 0
-        while done < args.steps:
-            n = min(CHUNK, args.steps - done)
+        while done < steps:
+            n = min(CHUNK, steps - done)
             kernel_ms += s.passoCronometrado(n)
             done += n
-            if done > args.steps - 1000:
+            if done > steps - 1000:
                 co = coeficientes(s.medirForcas(), c["units"], c["area"], g,
                                   corrigirBloqueio=True, theta=2.5)
                 jan.adicionar(co["cd"])
         s.sincronizar()
         wall = time.perf_counter() - t0
-        if args.campos:
-            s.salvarCampo(args.campos, c["u"])
+        if campos:
+            s.salvarCampo(campos, c["u"])
         mass, bad = s.massa()
     finally:
         s.destruir()
+    return jan.media, mass, bad, wall, kernel_ms, jan.desvio
+
+
+def main_sphere(args):
+    c = sphere_case()
+    g = c["grid"]
+    cells = g["nx"] * g["ny"] * g["nz"]
+    cfg = dict(omegaPlus=c["omega"], lesCs=0.1, inletU=[c["u"], 0, 0], esponja=c["esponja"])
+    cd, mass, bad, wall, kernel_ms, cd_std = run_sphere(c, args.precision, args.steps,
+                                                        max(args.warmup, 0), args.campos)
     cpu = None
     if args.cpu_seconds > 0:
         import oracle
@@ -305,7 +392,7 @@ def main_sphere(args):
                      "traffic": None, "avg_kernel_us": round(kernel_avg_s * 1e6, 2),
                      "note": "the 125 MB population set fits the 256 MB last-level cache"},
         "cpu_baseline": cpu,
-        "results": {"cd_mean_last_1000": jan.media, "cd_std": jan.desvio,
+        "results": {"cd_mean_last_1000": _finite(cd), "cd_std": _finite(cd_std),
                     "frontal_area_cells": c["area"], "mass_final": mass, "nonfinite_cells": bad,
                     "voxelize_ms": round(c["voxel_ms"], 2),
                     "campos": args.campos},
@@ -333,6 +420,7 @@ def main():
     bpc = BYTES_PER_CELL[precision]
     achieved = bpc * cells / kernel_avg_s / 1e9
     copy_gbs, _ = _lib.copy_bandwidth(1 << 30, 20)
+    prec = precision_check() if args.precision_check else None
     cpu = cpu_baseline(case, args.cpu_seconds) if args.cpu_seconds > 0 else None
     u = case["units"]
     line = {
@@ -373,14 +461,24 @@ def main():
         },
         "cpu_baseline": cpu,
         "results": {
-            "cd_mean_last_5000": r["cd"], "cd_std": r["cd_std"],
-            "cl_mean_last_5000": r["cl"], "cl_std": r["cl_std"], "force_samples": r["n_samples"],
+            "cd_mean": _finite(r["cd"]), "cd_std": _finite(r["cd_std"]),
+            "cl_mean": _finite(r["cl"]), "cl_std": _finite(r["cl_std"]),
+            "force_window": {
+                "samples": r["n_samples"], "first_step": r["first_sample"],
+                "last_step": r["last_step"], "every": CHUNK,
+                "ramp_steps": RAMP_STEPS,
+                # the configured measurement: mean of the last 5000 of 20000 steps
+                "as_configured": r["last_step"] >= 20000 and r["first_sample"] is not None
+                and r["last_step"] - r["first_sample"] + CHUNK >= WINDOW,
+                "transient": r["first_sample"] is None or r["first_sample"] <= RAMP_STEPS,
+            },
             "frontal_area_cells": case["area"], "solid_cells": case["cont"]["solido"],
             "voxelize_ms": case["vox"]["stats"]["msTotal"],
             "mass_final": r["mass"], "nonfinite_cells": r["nonfinite"],
             "copy_kernel_gbs": round(copy_gbs, 1),
             "frac_of_copy": round(achieved / copy_gbs, 4),
         },
+        "precision": prec,
     }
     print(json.dumps(line))
 

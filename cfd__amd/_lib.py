@@ -12,6 +12,9 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libcfdmi.so"
 
+# CFD_ABI_VERSION of include/cfd_mi355x.h that the structs below follow
+ABI_VERSION = 3
+
 CFD_STORE_FP32 = 0
 CFD_STORE_FP16 = 1
 CFD_ROUND_STOCHASTIC = 2
@@ -126,11 +129,18 @@ def _load() -> C.CDLL:
         "cfd_batch_destroy": (None, [P]),
         "cfd_last_error": (C.c_char_p, []),
         "cfd_version": (C.c_char_p, []),
+        "cfd_abi_version": (C.c_int, []),
     }
+    if not hasattr(lib, "cfd_abi_version"):
+        raise ImportError(f"{LIB_PATH} predates cfd_abi_version; rebuild it")
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    abi = lib.cfd_abi_version()
+    if abi != ABI_VERSION:
+        raise ImportError(f"{LIB_PATH} has struct layout version {abi}; these bindings "
+                          f"declare version {ABI_VERSION} (CFD_ABI_VERSION); rebuild it")
     return lib
 
 
@@ -142,7 +152,7 @@ EXPORTS = (
     "cfd_set_populations", "cfd_get_populations", "cfd_set_front", "cfd_get_front",
     "cfd_step", "cfd_step_timed", "cfd_set_tuning", "cfd_get_tuning", "cfd_forces", "cfd_update_macros", "cfd_read_macros",
     "cfd_mass", "cfd_synchronize", "cfd_bytes", "cfd_destroy", "cfd_voxelize",
-    "cfd_copy_bandwidth", "cfd_last_error", "cfd_version",
+    "cfd_copy_bandwidth", "cfd_last_error", "cfd_version", "cfd_abi_version",
     "cfd_set_closure", "cfd_set_wall_distance", "cfd_compute_wall_distance",
     "cfd_get_wall_distance", "cfd_closure_fallbacks", "cfd_mlp_train",
     "cfd_mlp_grad", "cfd_mlp_infer",

@@ -118,6 +118,15 @@ float dotc_host(int i, const float v[3]) {
 
 size_t elem_bytes(const cfd_solver* s) { return s->store == Store::F32 ? 4 : 2; }
 
+// 36 % of the step's launch grid (workgroups, padded to whole XCD chunk
+// spans as launch_step pads it).
+int default_rotation(size_t n, int block, int remap) {
+  const uint64_t nb = (n + block - 1) / block;
+  const uint64_t span = remap > 1 ? (uint64_t)kXcds * remap : 1;
+  const uint64_t grid = (nb + span - 1) / span * span;
+  return (int)(grid * 36 / 100);
+}
+
 }  // namespace
 
 namespace cfd {
@@ -166,6 +175,9 @@ StepParams make_step_params(const cfd_params& p) {
     volatile float r = ab - c;
     q.tinf[i] = wf(i) * r;
   }
+  q.sponge_all = 0;
+  for (int i = 0; i < Q; ++i)
+    if (!std::isfinite(q.tinf[i])) q.sponge_all = 1;
   return q;
 }
 
@@ -178,8 +190,10 @@ const char* cfd_last_error(void) { return g_err.c_str(); }
 // Lets the voxelizer translation unit report through cfd_last_error().
 void cfd__set_error(const char* msg) { g_err = msg ? msg : ""; }
 
-// 0.2: cfd_tuning gained rotate
-const char* cfd_version(void) { return "cfd_mi355x 0.2 gfx950"; }
+// 0.2: cfd_tuning gained rotate; 0.3: cfd_abi_version, default launch shape per set
+const char* cfd_version(void) { return "cfd_mi355x 0.3 gfx950"; }
+
+int cfd_abi_version(void) { return CFD_ABI_VERSION; }
 
 int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
   g_err.clear();
@@ -222,6 +236,17 @@ int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
   // the cache lost 1.5-2 % with it, one of 62 MB about 1 %.
   s->tuning.sweep = s->pop_bytes > (128ull << 20) &&
                     (s->store != Store::F32 || s->pop_bytes <= (256ull << 20));
+  // A float32 set larger than the cache (the 6.55 M-cell vehicle's 498 MB):
+  // 512-thread workgroups with the start rotated by 36 % of the launch grid
+  // per step, the launch-order autotuner's pick on every box measured
+  // (174.3-175.1 us per step on the vehicle against 175.5-177.5 us for
+  // 256-thread workgroups in ascending order).  Every shape gives the same
+  // bits, so this only sets what a run that does not autotune gets.
+  if (s->store == Store::F32 && s->pop_bytes > (256ull << 20) &&
+      s->tuning.layout == kLayoutBlocked) {
+    s->tuning.block = 512;
+    s->tuning.rotate = default_rotation(s->N, 512, s->tuning.remap);
+  }
 
   auto bail = [&](int code) {
     release(s);

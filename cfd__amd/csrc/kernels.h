@@ -46,6 +46,8 @@ struct StepParams {
   float bl, bl_m;                    // bl_m = max(bl, 1.0)   (wgsl.js:217)
   float tinf[19];   // free-stream equilibrium of the sponge  (ir.js:464-475)
   uint32_t sr_seed; // per-step seed of the stochastic float16 rounding
+  uint32_t sponge_all;  // some tinf is not finite: blend every fluid cell, as
+                        // mix(a, b, 0) = a*1 + b*0 is then not a (ir.js:477)
 };
 
 Geom make_geom(int nx, int ny, int nz, int layout, uint32_t slice_pad);

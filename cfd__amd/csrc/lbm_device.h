@@ -495,11 +495,13 @@ __device__ __forceinline__ void trt(const float (&gi)[Q], const float (&e)[Q], f
 }
 
 // Sponge (ir.js:458-478): mix(dst, teq, sigma^2).  For x <= start sigma^2 = 0
-// and mix returns dst unchanged, so the blend is skipped.
+// and mix returns dst unchanged while teq is finite, so the blend is skipped
+// there; a non-finite free stream (sponge_all, uniform) blends every fluid
+// cell as the reference's mix does.
 __device__ __forceinline__ void sponge(uint32_t x, const StepParams& p, float (&d)[Q]) {
   const float dxs = (float)x - p.sponge_start;
-  if (dxs > 0.0f) {
-    const float sigma = fminf(dxs / p.sponge_len_m, 1.0f);
+  if (dxs > 0.0f || p.sponge_all) {
+    const float sigma = fminf(fmaxf(dxs / p.sponge_len_m, 0.0f), 1.0f);
     const float sig2 = sigma * sigma;
 #pragma unroll
     for (int i = 0; i < Q; ++i) d[i] = d[i] * (1.0f - sig2) + p.tinf[i] * sig2;

@@ -44,6 +44,20 @@ def montar(seed: int = 7, celulas: float = 32, esteira: bool = True):
     return tipo, esponja
 
 
+def divergiu(s, limite: float = LIMITE_U, passo_amostra: int = 11) -> tuple[bool, float]:
+    """The page's divergence check (estabilidade.html:82-88): refresh the
+    macro field, sample |u| on every `passo_amostra`-th cell; diverged when a
+    sample is non-finite or above `limite`.  Returns (diverged, max |u|),
+    max |u| NaN when a sample is not finite."""
+    s.atualizarMacros()
+    m = s.lerMacros().reshape(-1, 4)[::passo_amostra, :3].astype(np.float64)
+    u = np.sqrt((m * m).sum(axis=1))
+    if not np.isfinite(u).all():
+        return True, float("nan")
+    max_u = float(u.max()) if u.size else 0.0
+    return max_u > limite, max_u
+
+
 def rodar_caso(s, omega: float, les: float, esponja: dict, passos: int = PASSOS,
                checar: int = 1000) -> dict:
     """One row of the page on an existing solver (types already set)."""
@@ -55,12 +69,8 @@ def rodar_caso(s, omega: float, les: float, esponja: dict, passos: int = PASSOS,
     while k < passos and morreu is None:
         s.passo(checar)
         k += checar
-        s.atualizarMacros()
-        m = s.lerMacros().reshape(-1, 4)[::11, :3].astype(np.float64)
-        u = np.sqrt((m * m).sum(axis=1))
-        ruim = not np.isfinite(u).all()
-        max_u = float(np.nanmax(np.where(np.isfinite(u), u, np.nan))) if not ruim else float("nan")
-        if ruim or max_u > LIMITE_U:
+        ruim, max_u = divergiu(s)
+        if ruim:
             morreu = s.passos
     nu = (1.0 / 3.0) * (1.0 / omega - 0.5)
     return {"omega": omega, "les": les, "nu": nu, "morreu": morreu, "maxU": max_u,

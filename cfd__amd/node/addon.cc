@@ -6,6 +6,7 @@
 #include <node_api.h>
 
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <string>
 
@@ -498,6 +499,15 @@ napi_value Version(napi_env env, napi_callback_info) {
 }  // namespace
 
 NAPI_MODULE_INIT() {
+  // the structs bound here follow this header's layout version
+  if (cfd_abi_version() != CFD_ABI_VERSION) {
+    char msg[160];
+    snprintf(msg, sizeof msg,
+             "libcfdmi has struct layout version %d; this addon was built for %d: rebuild",
+             cfd_abi_version(), CFD_ABI_VERSION);
+    napi_throw_error(env, nullptr, msg);
+    return nullptr;
+  }
   const struct {
     const char* name;
     napi_callback cb;

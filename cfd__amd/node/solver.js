@@ -148,10 +148,12 @@ class SolverMI355X {
   /** Launch-order autotuner (no reference counterpart): for each workgroup
    *  size, times `passos` live steps in ascending and in alternating cell
    *  order and with the step start rotated by each fraction of the launch
-   *  grid, keeps the fastest and returns the tuning.  The
-   *  blocos.length * (2 + fracoes.length) * (passos + 10) steps are ordinary
-   *  steps of the run: every order and shape gives the same bits. */
-  autoajustarPasso(passos = 150, fracoes = [0.02, 0.1, 0.36], blocos = [256, 512]) {
+   *  grid, keeps the fastest and returns the tuning.  Steps are taken in
+   *  calls of `bloco` steps (one settling call, then ceil(passos / bloco)
+   *  timed ones per candidate), as a caller's loop of passo(bloco) takes
+   *  them: the inflow ramp fires at the same steps and every order gives the
+   *  same bits, so the run equals one that calls passo(bloco) throughout. */
+  autoajustarPasso(passos = 200, fracoes = [0.02, 0.1, 0.36], blocos = [256, 512], bloco = 100) {
     const remap = this.ajuste.remap;
     const span = remap > 1 ? 8 * remap : 1;
     let cand = [];
@@ -160,12 +162,14 @@ class SolverMI355X {
       cand = cand.concat([{ block, sweep: 0, rotate: 0 }, { block, sweep: 1, rotate: 0 }],
         fracoes.map((f) => ({ block, sweep: 0, rotate: Math.max(1, Math.floor(f * grade)) })));
     }
+    const vezes = Math.max(1, Math.ceil(passos / bloco));
     let melhor = null;
     let melhorMs = Infinity;
     for (const c of cand) {
       this.ajuste = c;
-      this.passo(10);
-      const ms = this.passoCronometrado(passos);
+      this.passo(bloco);
+      let ms = 0;
+      for (let k = 0; k < vezes; ++k) ms += this.passoCronometrado(bloco);
       if (ms < melhorMs) { melhor = c; melhorMs = ms; }
     }
     this.ajuste = melhor;

@@ -186,16 +186,20 @@ class SolverMI355X:
         self.passos += n
         return ms.value
 
-    def autoajustarPasso(self, passos: int = 150, fracoes=(0.02, 0.1, 0.36),
-                         blocos=(256, 512)) -> dict:
+    def autoajustarPasso(self, passos: int = 200, fracoes=(0.02, 0.1, 0.36),
+                         blocos=(256, 512), bloco: int = 100) -> dict:
         """Launch-order autotuner (no reference counterpart): for each
         workgroup size, times `passos` steps in ascending and in alternating
         cell order and with the step start rotated by each fraction of the
-        launch grid, keeps the fastest and returns the tuning.  Every order and
-        shape updates every cell from the same previous state, so results are
-        bit-identical whichever is chosen, and the
-        len(blocos) * (2 + len(fracoes)) * (passos + 10) steps taken here are
-        ordinary steps of the run."""
+        launch grid, keeps the fastest and returns the tuning.
+
+        Steps are taken in calls of `bloco` steps (one settling call, then
+        ceil(passos / bloco) timed ones per candidate), as a caller's own loop
+        of passo(bloco) takes them: the inflow ramp advances and its callback
+        fires at the same steps, and every order and shape updates every cell
+        from the same previous state, so the run is bit-identical to one that
+        calls passo(bloco) throughout.  Total:
+        len(blocos) * (2 + len(fracoes)) * (1 + ceil(passos / bloco)) * bloco."""
         remap = self.ajuste["remap"]
         cand = []
         for block in blocos:
@@ -206,15 +210,22 @@ class SolverMI355X:
                      {"block": block, "sweep": 1, "rotate": 0}]
             cand += [{"block": block, "sweep": 0, "rotate": max(1, int(f * grade))}
                      for f in fracoes]
+        vezes = max(1, -(-passos // bloco))
         best, best_ms = None, float("inf")
         for c in cand:
             self.ajuste = c
-            self.passo(10)          # leave the previous order's cache state behind
-            ms = self.passoCronometrado(passos)
+            self.passo(bloco)          # leave the previous order's cache state behind
+            ms = sum(self.passoCronometrado(bloco) for _ in range(vezes))
             if ms < best_ms:
                 best, best_ms = c, ms
         self.ajuste = best
         return self.ajuste
+
+    @staticmethod
+    def passosAutoajuste(passos: int = 200, fracoes=(0.02, 0.1, 0.36), blocos=(256, 512),
+                         bloco: int = 100) -> int:
+        """Steps autoajustarPasso takes with these arguments."""
+        return len(blocos) * (2 + len(fracoes)) * (1 + max(1, -(-passos // bloco))) * bloco
 
     @property
     def ajuste(self) -> dict:

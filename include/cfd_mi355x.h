@@ -64,7 +64,11 @@ typedef struct cfd_params {
 } cfd_params;
 
 /* Launch shape and layout of the step kernel.  No reference counterpart:
- * the WGSL path has one fixed 64x1x1 workgroup (wgsl.js:186). */
+ * the WGSL path has one fixed 64x1x1 workgroup (wgsl.js:186).  cfd_create
+ * picks the shape measured fastest for the set: float32 sets above 256 MB
+ * get 512 threads with the start rotated by 36 % of the grid, float16 sets
+ * above 128 MB and float32 sets of 128-256 MB alternate the order, others
+ * run 256 threads in ascending order; all with XCD chunks of 8. */
 typedef struct cfd_tuning {
   int32_t block;        /* threads per workgroup: 256 or 512                   */
   int32_t remap;        /* 0/1: launch order; K (power of two): each XCD steps
@@ -161,7 +165,10 @@ int cfd_read_macros(cfd_solver *s, float *out, size_t n);
 
 /* Float64 mass-conservation reduction (the sum of delta that the reference's
  * nucleo.html:213-222 does on the host): sum over all non-solid cells of
- * sum_i g_i of the current front.  Also counts non-finite populations. */
+ * sum_i g_i of the current front, and the number of non-solid cells holding
+ * a non-finite population (left out of the sum): the divergence signal the
+ * reference's stability page reads from the macro field (estabilidade.html:
+ * 82-88). */
 int cfd_mass(cfd_solver *s, double *mass_out, int64_t *n_nonfinite_out);
 
 /* Blocks until all work queued on the handle's stream has finished. */
@@ -307,6 +314,16 @@ const char *cfd_last_error(void);
 
 /* Library build identification ("cfd_mi355x <version> gfx950"). */
 const char *cfd_version(void);
+
+/* Layout version of the structs this header declares (cfd_params,
+ * cfd_tuning, cfd_closure, cfd_tunnel_spec, ...).  A binding compares the
+ * CFD_ABI_VERSION of the header it was built against with cfd_abi_version()
+ * of the library it loads and refuses a mismatch, so that no caller passes a
+ * struct of another size.  No reference counterpart.
+ *   2: cfd_tuning without rotate (library 0.1)
+ *   3: cfd_tuning.rotate (library 0.2 and later) */
+#define CFD_ABI_VERSION 3
+int cfd_abi_version(void);
 
 #ifdef __cplusplus
 }

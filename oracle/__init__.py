@@ -88,8 +88,9 @@ def step(grid, src, tipo, params, dtype=np.float32, macros=False):
 
 
 def run(grid, src, tipo, params, nsteps, dtype=np.float32):
-    """nsteps A/B steps; returns (populations, macros of the last step)."""
-    a = np.ascontiguousarray(src, dtype=dtype)
+    """nsteps A/B steps; returns (populations, macros of the last step).
+    `src` is not modified."""
+    a = np.array(src, dtype=dtype, copy=True, order="C")
     b = a.copy()
     nx, ny, nz = grid
     t = np.ascontiguousarray(tipo, dtype=np.uint32)

@@ -38,6 +38,20 @@ def test_version_string():
     assert "gfx950" in _lib.version()
 
 
+def test_struct_layout_version_matches_the_header():
+    """The Python structs, the header's CFD_ABI_VERSION and the library agree,
+    and the loader refuses a library of another layout version."""
+    from cfd__amd import _lib
+    m = re.search(r"#define CFD_ABI_VERSION (\d+)", HEADER.read_text())
+    assert m and int(m.group(1)) == _lib.ABI_VERSION == _lib.lib.cfd_abi_version()
+    assert C.sizeof(_lib.Tuning) == 6 * 4          # cfd_tuning with rotate
+    assert C.sizeof(_lib.Params) == 12 * 8
+    src = (ROOT / "cfd__amd" / "_lib.py").read_text()
+    assert "abi != ABI_VERSION" in src
+    addon = (ROOT / "cfd__amd" / "node" / "addon.cc").read_text()
+    assert "cfd_abi_version() != CFD_ABI_VERSION" in addon
+
+
 def test_argument_errors_without_device_work():
     """Argument validation happens before any device call."""
     from cfd__amd import _lib

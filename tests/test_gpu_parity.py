@@ -305,6 +305,10 @@ def test_default_sweep_and_tuning_errors():
         s = S.criar(g, precisao=prec)
         try:
             assert s.ajuste["sweep"] == want, (g, prec)
+            # float32 sets above the 256 MB cache: 512 threads, start rotated
+            big32 = prec == "fp32" and g["nx"] * g["ny"] * g["nz"] * 76 > 256 << 20
+            assert s.ajuste["block"] == (512 if big32 else 256), (g, prec)
+            assert (s.ajuste["rotate"] > 0) == big32, (g, prec)
             for bad in ({"sweep": 2}, {"block": 128}, {"remap": 3}, {"layout": 2},
                         {"sweep": 0, "rotate": -1}, {"sweep": 1, "rotate": 5}):
                 with pytest.raises(CFDError):
@@ -360,31 +364,43 @@ def test_step_parity_soa_beyond_4gib(orc):
 
 
 def test_launch_order_autotuner_keeps_the_trajectory():
-    """autoajustarPasso times the candidate launch orders on live steps and
-    keeps the fastest; since every order is bit-identical, the state after
-    the tuning steps plus 20 more equals plain stepping for as many steps."""
+    """autoajustarPasso times the candidate launch orders on live steps, in
+    calls of the caller's chunk size, and keeps the fastest; since every
+    order is bit-identical and the inflow ramp fires at the same steps, the
+    state after the tuning steps plus 20 more equals a loop of passo(chunk)
+    for as many steps, with the ramp running through both."""
     grid = (72, 40, 36)
     tipo, esponja = _tunnel_types(grid, esteira=True)
     g0 = _random_state(grid, 5)
-    cfg = dict(omegaPlus=1.8, lesCs=0.1, inletU=[0.05, 0.0, 0.0], beltU=[0.05, 0.0, 0.0],
-               esponja=esponja, espessuraCL=4.0)
+    cfg = dict(omegaPlus=1.8, lesCs=0.1, beltU=[0.05, 0.0, 0.0], esponja=esponja,
+               espessuraCL=4.0)
+    chunk = 6
     out = []
     for tune in (True, False):
         s = _solver().criar({"nx": grid[0], "ny": grid[1], "nz": grid[2]})
         try:
             s.definirTipos(tipo)
-            s.configurar(**cfg)
+
+            def aplicar(f, s=s):
+                s.configurar(inletU=[0.05 * f, 0.0, 0.0], **cfg)
+
+            aplicar(1.0)
             s.frente = "A"
             s.definirPopulacoes(g0)
+            s.rampa(400, aplicar)
+            total = s.passosAutoajuste(passos=12, bloco=chunk)
+            assert total == 10 * 3 * chunk
             if tune:
-                t = s.autoajustarPasso(passos=12)
+                t = s.autoajustarPasso(passos=12, bloco=chunk)
                 assert t["sweep"] in (0, 1) and t["rotate"] >= 0
                 assert t["sweep"] == 0 or t["rotate"] == 0
-                assert s.passos == 10 * (12 + 10)
+                assert s.passos == total
                 assert t["block"] in (256, 512)
-                s.passo(20)
             else:
-                s.passo(10 * (12 + 10) + 20)
+                for _ in range(total // chunk):
+                    s.passo(chunk)
+            assert 0.0 < s.fatorRampa < 1.0
+            s.passo(20)
             out.append(s.lerPopulacoes())
         finally:
             s.destruir()

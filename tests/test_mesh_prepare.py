@@ -165,7 +165,7 @@ def _cube_stl() -> bytes:
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("nome", ["cube_example.stl", "vehicle.stl", "vehicle.glb"])
-def test_geometry_page(nome):
+def test_geometry_page(orc, nome):
     """tests/geometria.html:89-160 on the meshes at hand (the page's car files
     are not in the reference tree; nodes.glb is a transform fixture of four
     loose tetrahedra, not a body): read, place on the 'baixa' preset
@@ -178,7 +178,12 @@ def test_geometry_page(nome):
     m = mesh_io.ler(buf, nome)
     g = {"nx": 240, "ny": 120, "nz": 96}
     p = preparo.preparar(m, g, {"celulasPorComprimento": round(240 / 10)})
-    v = voxelizar(p["positions"], p["indices"], g, comSdf=False)
+    v = voxelizar(p["positions"], p["indices"], g, comSdf=True)
+    # read -> place -> GPU voxelizer equals the oracle's voxelizer on the same
+    # placed mesh, types and signed distance bit for bit
+    t_ref, sdf_ref, _ = orc.voxelize(p["positions"], p["indices"], (240, 120, 96))
+    assert np.array_equal(v["tipo"], t_ref), nome
+    assert np.array_equal(v["sdf"].view(np.uint32), sdf_ref.view(np.uint32)), nome
     s = v["stats"]
     af = tunel.areaFrontal(v["tipo"], g)
     assert s["nSolido"] > 200 and s["fracaoSolida"] < 0.35 and af > 50, (nome, s, af)

@@ -194,8 +194,10 @@ def test_node_closure_matches_python(tmp_path):
 
 @pytest.mark.gpu
 def test_node_launch_order_autotuner_keeps_the_trajectory(tmp_path):
-    """SolverMI355X.autoajustarPasso in Node: the state after the tuning
-    steps plus 20 more equals plain stepping for as many steps, bit for bit."""
+    """SolverMI355X.autoajustarPasso in Node: with the inflow ramp running,
+    the state after the tuning steps (taken in calls of the caller's chunk)
+    plus 20 more equals a loop of passo(chunk) for as many steps, bit for
+    bit."""
     grade = {"nx": 72, "ny": 40, "nz": 36}
     tipo, esponja, _ = tunel.montarTipos(grade, None, esteira=True)
     f = tmp_path / "tipo.u32"
@@ -213,11 +215,14 @@ const {{ SolverMI355X }} = require('./cfd__amd/node/solver.js');
   for (const tune of [true, false]) {{
     const s = await SolverMI355X.criar(ctx, grade);
     s.definirTipos(tipo);
-    s.configurar({{ omegaPlus: 1.8, lesCs: 0.1, inletU: [0.05, 0, 0], beltU: [0.05, 0, 0],
-                   esponja: {json.dumps(esponja)} }});
+    const aplicar = (f) => s.configurar({{ omegaPlus: 1.8, lesCs: 0.1, inletU: [0.05 * f, 0, 0],
+                   beltU: [0.05, 0, 0], esponja: {json.dumps(esponja)} }});
+    aplicar(1);
+    s.rampa(400, aplicar);
     s.inicializar();
-    if (tune) {{ escolha = s.autoajustarPasso(12); s.passo(20); }}
-    else s.passo(10 * 22 + 20);
+    if (tune) escolha = s.autoajustarPasso(12, [0.02, 0.1, 0.36], [256, 512], 6);
+    else for (let k = 0; k < 30; ++k) s.passo(6);
+    s.passo(20);
     out.push({{ p: await s.lerPopulacoes(), passos: s.passos }});
     s.destruir();
   }}
@@ -230,6 +235,6 @@ const {{ SolverMI355X }} = require('./cfd__amd/node/solver.js');
 """
     r = json.loads(_node(code).strip().splitlines()[-1])
     assert r["diff"] == 0 and r["n"] == 19 * 72 * 40 * 36, r
-    assert r["passos"] == [240, 240], r
+    assert r["passos"] == [200, 200], r
     e = r["escolha"]
     assert e["block"] in (256, 512) and e["sweep"] in (0, 1) and e["rotate"] >= 0 and (e["sweep"] == 0 or e["rotate"] == 0), e
