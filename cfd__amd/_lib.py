@@ -107,6 +107,9 @@ def _load() -> C.CDLL:
         "cfd_destroy": (None, [P]),
         "cfd_voxelize": (C.c_int, [P, C.c_size_t, P, C.c_size_t, C.c_int, C.c_int, C.c_int,
                                    C.c_double, C.c_int, P, P, C.POINTER(VoxStats)]),
+        "cfd_voxelize_noalloc": (C.c_int, [P, C.c_size_t, P, C.c_size_t, C.c_int, C.c_int,
+                                           C.c_int, C.c_double, C.c_int, P, P,
+                                           C.POINTER(VoxStats), P]),
         "cfd_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]),
         "cfd_set_closure": (C.c_int, [P, C.POINTER(Closure)]),
@@ -152,6 +155,7 @@ EXPORTS = (
     "cfd_set_populations", "cfd_get_populations", "cfd_set_front", "cfd_get_front",
     "cfd_step", "cfd_step_timed", "cfd_set_tuning", "cfd_get_tuning", "cfd_forces", "cfd_update_macros", "cfd_read_macros",
     "cfd_mass", "cfd_synchronize", "cfd_bytes", "cfd_destroy", "cfd_voxelize",
+    "cfd_voxelize_noalloc",
     "cfd_copy_bandwidth", "cfd_last_error", "cfd_version", "cfd_abi_version",
     "cfd_set_closure", "cfd_set_wall_distance", "cfd_compute_wall_distance",
     "cfd_get_wall_distance", "cfd_closure_fallbacks", "cfd_mlp_train",

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "../../include/cfd_mi355x.h"
@@ -508,6 +509,139 @@ int vfail(int code, const char* fmt, ...) {
   cfd__set_error(buf);
   return code;
 }
+__global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    p[k] = v;
+}
+
+// Device buffers of one voxelization.  pos/idx: the mesh; tipo (and sdf,
+// nullable): the result; the rest is scratch: flags, barrier, ext, ext2 (N
+// bytes each), dist2 (N words, only with sdf), counters (3), changed (1),
+// tile_chg (2 bytes per flood tile).
+struct VoxBuffers {
+  const float* pos;
+  const uint32_t* idx;
+  uint8_t *flags, *barrier, *ext, *ext2, *tipo;
+  uint32_t* dist2;
+  float* sdf;
+  unsigned long long* counters;  // [0] casca [1] dentro [2] triangles outside
+  int* changed;
+  uint8_t* tile_chg;
+};
+
+unsigned flood_tiles(int nx, int ny, int nz) {
+  return (unsigned)(((nx + FX - 1) / FX) * ((ny + FY - 1) / FY) * ((nz + FZ - 1) / FZ));
+}
+
+hipError_t fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t st) {
+  const size_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0,
+                     st, p, n, v);
+  return hipGetLastError();
+}
+
+// The whole pipeline on `st`, shared by both entry points: clear the scratch,
+// rasterize, seal, flood to the fixpoint (reading the change flag back after
+// each sweep), recover the skin and write the types.  ev (nullable): four
+// events recorded around raster / flood / final; iters: flood sweeps.
+hipError_t run_voxelize(const VoxBuffers& b, size_t nv, size_t nt, int nx, int ny, int nz,
+                        double band, int seal, hipStream_t st, hipEvent_t* ev, int* iters) {
+  const size_t N = (size_t)nx * ny * nz;
+  const unsigned cgrid = (unsigned)((N + kCellBlock - 1) / kCellBlock);
+  const unsigned tgrid = flood_tiles(nx, ny, nz);
+  hipError_t e;
+#define VRET(expr)                 \
+  do {                             \
+    if ((e = (expr)) != hipSuccess) \
+      return e;                    \
+  } while (0)
+  VRET(fill_u8(b.flags, N, 0, st));
+  VRET(fill_u8(reinterpret_cast<uint8_t*>(b.counters), 3 * sizeof(unsigned long long), 0, st));
+  if (b.dist2) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, st, b.dist2, N, INF_BITS);
+    VRET(hipGetLastError());
+  }
+  if (ev) VRET(hipEventRecord(ev[0], st));
+  if (nt) {
+    const size_t g = nt < 262144 ? nt : 262144;
+    hipLaunchKernelGGL(k_raster, dim3((unsigned)g), dim3(kRasterBlock), 0, st, b.pos, nv, b.idx,
+                       nt, nx, ny, nz, band, b.flags, b.dist2, b.counters + 2);
+    VRET(hipGetLastError());
+  }
+  if (ev) VRET(hipEventRecord(ev[1], st));
+  const uint8_t* bar = b.flags;
+  if (seal) {
+    // x pass into ext, y pass into ext2 (both rewritten before use), z pass
+    // into the barrier
+    hipLaunchKernelGGL(k_dilate_axis, dim3(cgrid), dim3(kCellBlock), 0, st, b.flags, b.ext, nx,
+                       ny, nz, 0, 1);
+    hipLaunchKernelGGL(k_dilate_axis, dim3(cgrid), dim3(kCellBlock), 0, st, b.ext, b.ext2, nx,
+                       ny, nz, 1, 0);
+    hipLaunchKernelGGL(k_dilate_axis, dim3(cgrid), dim3(kCellBlock), 0, st, b.ext2, b.barrier,
+                       nx, ny, nz, 2, 0);
+    VRET(hipGetLastError());
+    bar = b.barrier;
+  }
+  hipLaunchKernelGGL(k_seed, dim3(cgrid), dim3(kCellBlock), 0, st, bar, b.ext, nx, ny, nz);
+  VRET(hipGetLastError());
+  VRET(fill_u8(b.tile_chg, tgrid, 1, st));   // first sweep: every tile
+  int n_it = 0;
+  for (;;) {
+    VRET(fill_u8(reinterpret_cast<uint8_t*>(b.changed), sizeof(int), 0, st));
+    const uint8_t* prev = b.tile_chg + (size_t)(n_it & 1) * tgrid;
+    uint8_t* next = b.tile_chg + (size_t)((n_it + 1) & 1) * tgrid;
+    hipLaunchKernelGGL(k_flood_tile, dim3(tgrid), dim3(kCellBlock), 0, st, bar, b.ext, nx, ny,
+                       nz, prev, next, b.changed);
+    VRET(hipGetLastError());
+    int hchg = 0;
+    VRET(hipMemcpyAsync(&hchg, b.changed, sizeof(int), hipMemcpyDeviceToHost, st));
+    VRET(hipStreamSynchronize(st));
+    ++n_it;
+    if (!hchg) break;
+  }
+  uint8_t* ext_final = b.ext;
+  if (seal) {
+    hipLaunchKernelGGL(k_skin, dim3(cgrid), dim3(kCellBlock), 0, st, b.ext, b.flags, b.ext2, nx,
+                       ny, nz);
+    VRET(hipGetLastError());
+    ext_final = b.ext2;
+  }
+  if (ev) VRET(hipEventRecord(ev[2], st));
+  hipLaunchKernelGGL(k_final, dim3(cgrid < 2048u ? cgrid : 2048u), dim3(kCellBlock), 0, st,
+                     b.flags, ext_final, b.dist2, N, band, b.tipo, b.sdf, b.counters);
+  VRET(hipGetLastError());
+  if (ev) VRET(hipEventRecord(ev[3], st));
+#undef VRET
+  *iters = n_it;
+  return hipSuccess;
+}
+
+void fill_stats(const unsigned long long hc[3], size_t N, int iters, float ms_raster,
+                float ms_flood, float ms_total, cfd_vox_stats* stats) {
+  stats->n_cells = (int64_t)N;
+  stats->n_casca = (int64_t)hc[0];
+  stats->n_dentro = (int64_t)hc[1];
+  stats->n_solido = (int64_t)(hc[0] + hc[1]);
+  stats->triangles_outside = (int64_t)hc[2];
+  stats->hollow_suspect = (hc[1] < hc[0] * 0.25 && hc[0] > 500) ? 1 : 0;
+  stats->flood_iterations = iters;
+  stats->ms_raster = ms_raster;
+  stats->ms_flood = ms_flood;
+  stats->ms_total = ms_total;
+}
+
+int check_args(const char* fn, const void* pos, size_t nv, const void* idx, size_t nt, int nx,
+               int ny, int nz, double band, const void* tipo_out) {
+  if (nx < 1 || ny < 1 || nz < 1)
+    return vfail(CFD_E_ARG, "%s: grid %dx%dx%d must be positive", fn, nx, ny, nz);
+  if (!tipo_out) return vfail(CFD_E_ARG, "%s: tipo_out is null", fn);
+  if ((nv && !pos) || (nt && !idx)) return vfail(CFD_E_ARG, "%s: null mesh arrays", fn);
+  if (!std::isfinite(band)) return vfail(CFD_E_ARG, "%s: band must be finite", fn);
+  if ((uint64_t)nx * ny * nz >= (1ull << 31)) return vfail(CFD_E_ARG, "%s: grid too large", fn);
+  return CFD_OK;
+}
+
 }  // namespace
 
 #define VCK(expr)                                                                    \
@@ -524,138 +658,151 @@ extern "C" int cfd_voxelize(const float* pos, size_t nv, const uint32_t* idx, si
                             int nx, int ny, int nz, double band, int seal, uint8_t* tipo_out,
                             float* sdf_out, cfd_vox_stats* stats) {
   cfd__set_error("");
-  if (nx < 1 || ny < 1 || nz < 1)
-    return vfail(CFD_E_ARG, "cfd_voxelize: grid %dx%dx%d must be positive", nx, ny, nz);
-  if (!tipo_out) return vfail(CFD_E_ARG, "cfd_voxelize: tipo_out is null");
-  if ((nv && !pos) || (nt && !idx)) return vfail(CFD_E_ARG, "cfd_voxelize: null mesh arrays");
-  if (!std::isfinite(band)) return vfail(CFD_E_ARG, "cfd_voxelize: band must be finite");
+  int rc = check_args("cfd_voxelize", pos, nv, idx, nt, nx, ny, nz, band, tipo_out);
+  if (rc) return rc;
   const size_t N = (size_t)nx * ny * nz;
-  if (N >= (1ull << 31)) return vfail(CFD_E_ARG, "cfd_voxelize: grid too large");
-
-  int rc = CFD_OK;
   hipStream_t st = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float* d_pos = nullptr;
   uint32_t* d_idx = nullptr;
-  uint8_t *flags = nullptr, *barrier = nullptr, *ext = nullptr, *ext2 = nullptr,
-          *tipo = nullptr;
-  uint32_t* dist2 = nullptr;
-  float* sdf = nullptr;
-  unsigned long long* counters = nullptr;  // [0] casca [1] dentro [2] outside
-  int* changed = nullptr;
-  uint8_t* tile_chg = nullptr;  // two sweeps' per-tile change flags
+  VoxBuffers b{};
   unsigned long long hc[3] = {0, 0, 0};
   int iters = 0;
-  const unsigned cgrid = (unsigned)((N + kCellBlock - 1) / kCellBlock);
-  const unsigned tgrid =
-      (unsigned)(((nx + FX - 1) / FX) * ((ny + FY - 1) / FY) * ((nz + FZ - 1) / FZ));
+  const unsigned tgrid = flood_tiles(nx, ny, nz);
 
   VCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   for (auto& e : ev) VCK(hipEventCreate(&e));
   if (nv) VCK(hipMalloc(&d_pos, nv * 3 * sizeof(float)));
   if (nt) VCK(hipMalloc(&d_idx, nt * 3 * sizeof(uint32_t)));
-  VCK(hipMalloc(&flags, N));
-  VCK(hipMalloc(&barrier, N));
-  VCK(hipMalloc(&ext, N));
-  VCK(hipMalloc(&ext2, N));
-  VCK(hipMalloc(&tipo, N));
+  VCK(hipMalloc(&b.flags, N));
+  VCK(hipMalloc(&b.barrier, N));
+  VCK(hipMalloc(&b.ext, N));
+  VCK(hipMalloc(&b.ext2, N));
+  VCK(hipMalloc(&b.tipo, N));
   // the float32 distance field (atomic minimum per cell) only feeds the
   // signed distance; without sdf_out the rasterizer skips it
-  if (sdf_out) VCK(hipMalloc(&dist2, N * sizeof(uint32_t)));
-  if (sdf_out) VCK(hipMalloc(&sdf, N * sizeof(float)));
-  VCK(hipMalloc(&counters, 3 * sizeof(unsigned long long)));
-  VCK(hipMalloc(&changed, sizeof(int)));
-  VCK(hipMalloc(&tile_chg, 2 * (size_t)tgrid));
+  if (sdf_out) VCK(hipMalloc(&b.dist2, N * sizeof(uint32_t)));
+  if (sdf_out) VCK(hipMalloc(&b.sdf, N * sizeof(float)));
+  VCK(hipMalloc(&b.counters, 3 * sizeof(unsigned long long)));
+  VCK(hipMalloc(&b.changed, sizeof(int)));
+  VCK(hipMalloc(&b.tile_chg, 2 * (size_t)tgrid));
   if (nv) VCK(hipMemcpyAsync(d_pos, pos, nv * 3 * sizeof(float), hipMemcpyHostToDevice, st));
   if (nt) VCK(hipMemcpyAsync(d_idx, idx, nt * 3 * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-  VCK(hipMemsetAsync(flags, 0, N, st));
-  VCK(hipMemsetAsync(counters, 0, 3 * sizeof(unsigned long long), st));
-  if (dist2) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, st, dist2, N, INF_BITS);
-    VCK(hipGetLastError());
-  }
-
-  VCK(hipEventRecord(ev[0], st));
-  if (nt) {
-    const size_t g = nt < 262144 ? nt : 262144;
-    hipLaunchKernelGGL(k_raster, dim3((unsigned)g), dim3(kRasterBlock), 0, st, d_pos, nv, d_idx,
-                       nt, nx, ny, nz, band, flags, dist2, counters + 2);
-    VCK(hipGetLastError());
-  }
-  VCK(hipEventRecord(ev[1], st));
-
-  {
-    const uint8_t* bar = flags;
-    if (seal) {
-      // x pass into ext, y pass into ext2 (both rewritten before use), z pass
-      // into the barrier
-      hipLaunchKernelGGL(k_dilate_axis, dim3(cgrid), dim3(kCellBlock), 0, st, flags, ext, nx,
-                         ny, nz, 0, 1);
-      hipLaunchKernelGGL(k_dilate_axis, dim3(cgrid), dim3(kCellBlock), 0, st, ext, ext2, nx,
-                         ny, nz, 1, 0);
-      hipLaunchKernelGGL(k_dilate_axis, dim3(cgrid), dim3(kCellBlock), 0, st, ext2, barrier, nx,
-                         ny, nz, 2, 0);
-      VCK(hipGetLastError());
-      bar = barrier;
-    }
-    hipLaunchKernelGGL(k_seed, dim3(cgrid), dim3(kCellBlock), 0, st, bar, ext, nx, ny, nz);
-    VCK(hipGetLastError());
-    VCK(hipMemsetAsync(tile_chg, 1, tgrid, st));   // first sweep: every tile
-    for (;;) {
-      VCK(hipMemsetAsync(changed, 0, sizeof(int), st));
-      const uint8_t* prev = tile_chg + (size_t)(iters & 1) * tgrid;
-      uint8_t* next = tile_chg + (size_t)((iters + 1) & 1) * tgrid;
-      hipLaunchKernelGGL(k_flood_tile, dim3(tgrid), dim3(kCellBlock), 0, st, bar, ext, nx, ny,
-                         nz, prev, next, changed);
-      VCK(hipGetLastError());
-      int hchg = 0;
-      VCK(hipMemcpyAsync(&hchg, changed, sizeof(int), hipMemcpyDeviceToHost, st));
-      VCK(hipStreamSynchronize(st));
-      ++iters;
-      if (!hchg) break;
-    }
-    uint8_t* ext_final = ext;
-    if (seal) {
-      hipLaunchKernelGGL(k_skin, dim3(cgrid), dim3(kCellBlock), 0, st, ext, flags, ext2, nx, ny,
-                         nz);
-      VCK(hipGetLastError());
-      ext_final = ext2;
-    }
-    VCK(hipEventRecord(ev[2], st));
-    hipLaunchKernelGGL(k_final, dim3(cgrid < 2048u ? cgrid : 2048u), dim3(kCellBlock), 0, st,
-                       flags, ext_final, dist2, N, band, tipo, sdf, counters);
-    VCK(hipGetLastError());
-  }
-  VCK(hipEventRecord(ev[3], st));
-  VCK(hipMemcpyAsync(tipo_out, tipo, N, hipMemcpyDeviceToHost, st));
-  if (sdf_out) VCK(hipMemcpyAsync(sdf_out, sdf, N * sizeof(float), hipMemcpyDeviceToHost, st));
-  VCK(hipMemcpyAsync(hc, counters, sizeof hc, hipMemcpyDeviceToHost, st));
+  b.pos = d_pos;
+  b.idx = d_idx;
+  VCK(run_voxelize(b, nv, nt, nx, ny, nz, band, seal, st, ev, &iters));
+  VCK(hipMemcpyAsync(tipo_out, b.tipo, N, hipMemcpyDeviceToHost, st));
+  if (sdf_out) VCK(hipMemcpyAsync(sdf_out, b.sdf, N * sizeof(float), hipMemcpyDeviceToHost, st));
+  VCK(hipMemcpyAsync(hc, b.counters, sizeof hc, hipMemcpyDeviceToHost, st));
   VCK(hipStreamSynchronize(st));
   if (stats) {
     float t01 = 0, t12 = 0, t03 = 0;
     VCK(hipEventElapsedTime(&t01, ev[0], ev[1]));
     VCK(hipEventElapsedTime(&t12, ev[1], ev[2]));
     VCK(hipEventElapsedTime(&t03, ev[0], ev[3]));
-    stats->n_cells = (int64_t)N;
-    stats->n_casca = (int64_t)hc[0];
-    stats->n_dentro = (int64_t)hc[1];
-    stats->n_solido = (int64_t)(hc[0] + hc[1]);
-    stats->triangles_outside = (int64_t)hc[2];
-    stats->hollow_suspect = (hc[1] < hc[0] * 0.25 && hc[0] > 500) ? 1 : 0;
-    stats->flood_iterations = iters;
-    stats->ms_raster = t01;
-    stats->ms_flood = t12;
-    stats->ms_total = t03;
+    fill_stats(hc, N, iters, t01, t12, t03, stats);
   }
 
 done:
   if (st) (void)hipStreamSynchronize(st);
-  for (void* p : {(void*)d_pos, (void*)d_idx, (void*)flags, (void*)barrier, (void*)ext,
-                  (void*)ext2, (void*)tipo, (void*)dist2, (void*)sdf, (void*)counters,
-                  (void*)changed, (void*)tile_chg})
+  for (void* p : {(void*)d_pos, (void*)d_idx, (void*)b.flags, (void*)b.barrier, (void*)b.ext,
+                  (void*)b.ext2, (void*)b.tipo, (void*)b.dist2, (void*)b.sdf,
+                  (void*)b.counters, (void*)b.changed, (void*)b.tile_chg})
     if (p) (void)hipFree(p);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (st) (void)hipStreamDestroy(st);
   return rc;
+}
+#undef VCK
+
+// ---- the same voxelization without allocation ----
+// Scratch in one fixed device array per GPU: flags, barrier, ext, ext2 (one
+// byte per cell each), dist2 (a word per cell), counters, the change flag and
+// the flood tiles' change bytes, at fixed offsets sized for
+// CFD_VOXELIZE_NOALLOC_MAX_CELLS cells and kNoAllocMaxTiles flood tiles.
+namespace {
+constexpr size_t kNoAllocMaxCells = CFD_VOXELIZE_NOALLOC_MAX_CELLS;
+constexpr size_t kNoAllocMaxTiles = 1u << 18;
+constexpr size_t kOffCounters = 8 * kNoAllocMaxCells;  // after 4 byte planes + dist2
+constexpr size_t kOffChanged = kOffCounters + 256;
+constexpr size_t kOffTiles = kOffChanged + 256;
+constexpr size_t kNoAllocBytes = kOffTiles + 2 * kNoAllocMaxTiles;
+static_assert(kNoAllocMaxCells % 256 == 0, "byte planes keep 256-byte alignment");
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array at its own address).
+int scratch_of_current_device(uint8_t** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+    return vfail(CFD_E_HIP, "cfd_voxelize_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices)
+    return vfail(CFD_E_HIP, "cfd_voxelize_noalloc: device %d outside 0..%d", dev,
+                 kMaxDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return vfail(CFD_E_HIP, "cfd_voxelize_noalloc: hipGetSymbolAddress(scratch): %s",
+                   hipGetErrorString(e));
+    g_scratch[dev] = static_cast<uint8_t*>(p);
+  }
+  *out = g_scratch[dev];
+  return CFD_OK;
+}
+}  // namespace
+
+extern "C" int cfd_voxelize_noalloc(const float* d_pos, size_t nv, const uint32_t* d_idx,
+                                    size_t nt, int nx, int ny, int nz, double band, int seal,
+                                    uint8_t* d_tipo_out, float* d_sdf_out,
+                                    cfd_vox_stats* stats, void* stream) {
+  cfd__set_error("");
+  int rc = check_args("cfd_voxelize_noalloc", d_pos, nv, d_idx, nt, nx, ny, nz, band,
+                      d_tipo_out);
+  if (rc) return rc;
+  const size_t N = (size_t)nx * ny * nz;
+  if (N > kNoAllocMaxCells)
+    return vfail(CFD_E_ARG,
+                 "cfd_voxelize_noalloc: %zu cells exceed the %zu cells of its fixed scratch "
+                 "(cfd_voxelize has no such limit)", N, kNoAllocMaxCells);
+  const unsigned tgrid = flood_tiles(nx, ny, nz);
+  if (tgrid > kNoAllocMaxTiles)
+    return vfail(CFD_E_ARG,
+                 "cfd_voxelize_noalloc: grid %dx%dx%d needs %u flood tiles of 32x16x16 cells, "
+                 "more than the %zu of its fixed scratch", nx, ny, nz, tgrid, kNoAllocMaxTiles);
+  uint8_t* base = nullptr;
+  rc = scratch_of_current_device(&base);
+  if (rc) return rc;
+  VoxBuffers b{};
+  b.pos = d_pos;
+  b.idx = d_idx;
+  b.flags = base;
+  b.barrier = base + kNoAllocMaxCells;
+  b.ext = base + 2 * kNoAllocMaxCells;
+  b.ext2 = base + 3 * kNoAllocMaxCells;
+  b.dist2 = d_sdf_out ? reinterpret_cast<uint32_t*>(base + 4 * kNoAllocMaxCells) : nullptr;
+  b.tipo = d_tipo_out;
+  b.sdf = d_sdf_out;
+  b.counters = reinterpret_cast<unsigned long long*>(base + kOffCounters);
+  b.changed = reinterpret_cast<int*>(base + kOffChanged);
+  b.tile_chg = base + kOffTiles;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int iters = 0;
+  unsigned long long hc[3] = {0, 0, 0};
+  hipError_t e = run_voxelize(b, nv, nt, nx, ny, nz, band, seal, st, nullptr, &iters);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(hc, b.counters, sizeof hc, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess)
+    return vfail(CFD_E_HIP, "cfd_voxelize_noalloc: %s", hipGetErrorString(e));
+  if (stats) fill_stats(hc, N, iters, 0.0f, 0.0f, 0.0f, stats);
+  return CFD_OK;
 }

@@ -304,6 +304,26 @@ int cfd_voxelize(const float *pos, size_t nv, const uint32_t *idx, size_t nt,
                  int nx, int ny, int nz, double band, int seal,
                  uint8_t *tipo_out, float *sdf_out, cfd_vox_stats *stats);
 
+/* cfd_voxelize for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * No counterpart in the reference.  The mesh (d_pos, d_idx) and the results
+ * (d_tipo_out: N bytes, d_sdf_out: N floats, nullable) are DEVICE pointers
+ * owned by the caller; the work runs on `stream` (a hipStream_t; NULL is the
+ * null stream), which the call synchronizes before it returns (the flood
+ * reads its change flag back after every sweep).  It calls no hipMalloc,
+ * hipFree, stream or event creation: its scratch is one fixed device array
+ * of 64 MiB + 512 KiB + 512 B per GPU, held from library load to process
+ * end, which limits the grid to CFD_VOXELIZE_NOALLOC_MAX_CELLS cells (and
+ * 2^18 flood tiles of 32x16x16 cells); larger grids fail with CFD_E_ARG.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * race.  Results are bit-identical to cfd_voxelize; stats carries the counts
+ * and flood_iterations, its ms fields are 0 (no events are created). */
+#define CFD_VOXELIZE_NOALLOC_MAX_CELLS 8388608 /* 2^23: 320x160x128 fits */
+int cfd_voxelize_noalloc(const float *d_pos, size_t nv, const uint32_t *d_idx, size_t nt,
+                         int nx, int ny, int nz, double band, int seal,
+                         uint8_t *d_tipo_out, float *d_sdf_out, cfd_vox_stats *stats,
+                         void *stream);
+
 /* Achievable-HBM probe: a plain device-to-device copy kernel over `bytes`
  * bytes, `iters` times; returns the mean copy bandwidth (read+write GB/s).
  * Used to calibrate the roofline; no reference counterpart. */
