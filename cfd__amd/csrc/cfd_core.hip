@@ -70,6 +70,15 @@ struct cfd_solver {
   uint8_t* wall8 = nullptr;   // wall distance, 1/8 cell, saturating at 31.875
   bool wall_given = false;    // wall8 holds the caller's distances, not the types' transform
   unsigned long long* fallbacks = nullptr;
+  // The macro field is computed when something reads it, not in every
+  // batch of steps: a step records the set it read and its parameters, and
+  // materialize_macros() pulls that step's moments from that set, bit for
+  // bit what the step would have stored (the same pull and moments code).
+  // Anything that overwrites that set, the types or the layout first
+  // materializes a pending field.
+  bool macros_pending = false;
+  int macros_src = 0;
+  StepParams macros_params{};
 };
 
 namespace {
@@ -334,12 +343,15 @@ int build_links(cfd_solver* s) {
 }  // namespace
 
 static int compute_wall(cfd_solver* s);
+static int materialize_macros(cfd_solver* s);
 
 int cfd_set_types(cfd_solver* s, const uint32_t* tipo, size_t n) {
   g_err.clear();
   if (!s || !tipo) return fail(CFD_E_ARG, "cfd_set_types: null argument");
   if (n != s->N)
     return fail(CFD_E_ARG, "cfd_set_types: type field has %zu cells, expected %zu", n, s->N);
+  const int mrc = materialize_macros(s);
+  if (mrc) return mrc;
   uint32_t* d32 = nullptr;
   HIPCK(hipMalloc(&d32, n * sizeof(uint32_t)));
   hipError_t e = hipMemcpyAsync(d32, tipo, n * sizeof(uint32_t), hipMemcpyHostToDevice,
@@ -359,6 +371,8 @@ int cfd_init_equilibrium(cfd_solver* s) {
   if (!s) return fail(CFD_E_ARG, "cfd_init_equilibrium: null handle");
   if (!s->configured)
     return fail(CFD_E_STATE, "cfd_init_equilibrium: cfd_configure was never called");
+  const int rc = materialize_macros(s);
+  if (rc) return rc;
   HIPCK(launch_init(s->store, s->pop[1], s->tipo8, s->geom, s->params, s->stream));
   s->front = 1;  // init writes B (solver.js:342-346)
   return CFD_OK;
@@ -396,6 +410,8 @@ int cfd_set_populations(cfd_solver* s, const float* soa, size_t n) {
   if (n != (size_t)Q * s->N)
     return fail(CFD_E_ARG, "cfd_set_populations: %zu values, expected %zu", n,
                 (size_t)Q * s->N);
+  const int rc = materialize_macros(s);
+  if (rc) return rc;
   return upload_dense(s, soa, s->pop[s->front]);
 }
 
@@ -434,6 +450,8 @@ int cfd_set_tuning(cfd_solver* s, const cfd_tuning* t) {
     return fail(CFD_E_ARG, "cfd_set_tuning: the blocked layout needs a population set below "
                            "4 GiB (this grid: %.2f GB)", ng.pop_elems * eb / 1e9);
   if (ng.lb != s->geom.lb || ng.S != s->geom.S) {
+    const int rc = materialize_macros(s);
+    if (rc) return rc;
     // Re-lay both population sets (contents preserved, walls included).
     const size_t n = (size_t)Q * s->N;
     float* dense = nullptr;
@@ -488,6 +506,8 @@ int cfd_set_front(cfd_solver* s, int front) {
   g_err.clear();
   if (!s) return fail(CFD_E_ARG, "cfd_set_front: null handle");
   if (front != 0 && front != 1) return fail(CFD_E_ARG, "cfd_set_front: front must be 0 or 1");
+  const int rc = materialize_macros(s);
+  if (rc) return rc;
   s->front = front;
   return CFD_OK;
 }
@@ -499,19 +519,31 @@ int cfd_get_front(cfd_solver* s, int* front) {
   return CFD_OK;
 }
 
+// The pending macro field of the last step into s->macros (see cfd_solver).
+static int materialize_macros(cfd_solver* s) {
+  if (!s->macros_pending) return CFD_OK;
+  HIPCK(launch_pull_macros(s->store, s->pop[s->macros_src], s->tipo8, s->macros, s->geom,
+                           s->macros_params, s->stream));
+  s->macros_pending = false;
+  return CFD_OK;
+}
+
 static int enqueue_steps(cfd_solver* s, int n) {
   for (int k = 0; k < n; ++k) {
     const int f = s->front;
     s->params.sr_seed = ++s->sr_step * 0x9E3779B9u;   // a new rounding draw per step
     if (s->mlp)
       HIPCK(launch_step_closure(s->store, s->pop[f], s->pop[f ^ 1], s->tipo8, s->wall8, s->mlp,
-                                s->macros, k == n - 1, s->geom, s->params, s->fallbacks,
-                                s->stream));
+                                s->macros, false, s->geom, s->params, s->fallbacks, s->stream));
     else
-      HIPCK(launch_step(s->store, s->pop[f], s->pop[f ^ 1], s->tipo8, s->macros, k == n - 1,
+      HIPCK(launch_step(s->store, s->pop[f], s->pop[f ^ 1], s->tipo8, s->macros, false,
                         s->geom, s->params, s->tuning, s->tuning.sweep && (s->sr_step & 1),
                         s->sr_step, s->stream));
     s->front = f ^ 1;
+    // this step's macro field supersedes any pending one
+    s->macros_pending = true;
+    s->macros_src = f;
+    s->macros_params = s->params;
   }
   return CFD_OK;
 }
@@ -557,6 +589,7 @@ int cfd_update_macros(cfd_solver* s) {
   g_err.clear();
   if (!s) return fail(CFD_E_ARG, "cfd_update_macros: null handle");
   HIPCK(launch_macros(s->store, s->pop[s->front], s->macros, s->geom, s->stream));
+  s->macros_pending = false;   // overwritten
   return CFD_OK;
 }
 
@@ -565,6 +598,8 @@ int cfd_read_macros(cfd_solver* s, float* out, size_t n) {
   if (!s || !out) return fail(CFD_E_ARG, "cfd_read_macros: null argument");
   if (n != 4 * s->N)
     return fail(CFD_E_ARG, "cfd_read_macros: %zu values, expected %zu", n, 4 * s->N);
+  int rc = materialize_macros(s);
+  if (rc) return rc;
   HIPCK(hipMemcpyAsync(out, s->macros, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
   HIPCK(hipStreamSynchronize(s->stream));
   return CFD_OK;

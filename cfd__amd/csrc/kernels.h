@@ -77,6 +77,9 @@ hipError_t launch_step(Store sto, const void* src, void* dst, const uint8_t* tip
                        uint64_t step, hipStream_t st);
 hipError_t launch_init(Store sto, void* dst, const uint8_t* tipo8, const Geom& g,
                        const StepParams& p, hipStream_t st);
+// The macro field a step that read `src` with `p` would have written.
+hipError_t launch_pull_macros(Store sto, const void* src, const uint8_t* tipo8, float4* macros,
+                              const Geom& g, const StepParams& p, hipStream_t st);
 hipError_t launch_macros(Store sto, const void* src, float4* macros, const Geom& g,
                          hipStream_t st);
 // Body links: one entry {cell, mask of directions i with a SOLIDO neighbour

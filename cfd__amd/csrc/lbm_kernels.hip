@@ -68,7 +68,12 @@ __global__ __launch_bounds__(kBlock) void k_prep_types(const uint32_t* __restric
 // being pulled (the previous step's post-collision populations): every pull
 // of a FLUIDO cell from a SOLIDO neighbour x + c_j re-reads g_j(x), which is
 // exactly the link term c_j * 2 (g_j + w_j) of wgsl.js:358-379.
-template <class S, bool MACROS, int LB, bool FORCE>
+//
+// PULL_ONLY = true stops after the moments: it writes only the macro field
+// (what MACROS stores), nothing into dst.  Run on the set a step read, with
+// the parameters that step used, it gives that step's macro field bit for
+// bit (the same pull, bounce-back and moments code).
+template <class S, bool MACROS, int LB, bool FORCE, bool PULL_ONLY = false>
 __device__ __forceinline__ void step_cell(uint32_t cell,
                                           const GLOBAL typename S::T* __restrict__ src,
                                           GLOBAL typename S::T* __restrict__ dst,
@@ -114,7 +119,7 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
   const bool mixed = __ballot(!wall) != 0;
   if (wall) {
     if (MACROS) macros[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (mixed) {
+    if (!PULL_ONLY && mixed) {
 #pragma unroll
       for (int i = 0; i < Q; ++i) out.st(i, own, 0.0f, 0u);
     }
@@ -125,6 +130,10 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
   // 2. moments, then the branch chain ESPELHO_Y -> ESPELHO_Z -> ENTRADA ->
   //    SAIDA -> fluid (equilibrium, Smagorinsky, TRT, sponge).
   const Moments M = moments(gi);
+  if (PULL_ONLY) {
+    macros[cell] = make_float4(M.ux, M.uy, M.uz, M.delta);
+    return;
+  }
   float d[Q];
   if (!boundary_cell(ct, gi, M, c.z, p, d)) {
     float e[Q];
@@ -160,6 +169,19 @@ __global__ __launch_bounds__(BLOCK) void k_step(const typename S::T* __restrict_
   if (cell >= g.N) return;
   step_cell<S, MACROS, LB, false>(cell, gptr(src), gptr(dst), gptr(tipo), macros, g, p,
                                   nullptr);
+}
+
+// The macro field of the step that read `src` with parameters `p`, without
+// collision or stores into a population set.
+template <class S, int LB>
+__global__ __launch_bounds__(kBlock) void k_pull_macros(const typename S::T* __restrict__ src,
+                                                        const uint8_t* __restrict__ tipo,
+                                                        float4* __restrict__ macros, Geom g,
+                                                        StepParams p) {
+  const uint32_t cell = logical_block(g) * kBlock + threadIdx.x;
+  if (cell >= g.N) return;
+  step_cell<S, true, LB, false, true>(cell, gptr(src), nullptr, gptr(tipo), macros, g, p,
+                                      nullptr);
 }
 
 // Workgroup sum of 3 doubles for any BLOCK (wave64 butterfly, then the
@@ -664,6 +686,15 @@ hipError_t launch_init(Store sto, void* dst, const uint8_t* tipo8, const Geom& g
   CFD_DISPATCH(sto, g, {
     hipLaunchKernelGGL((k_init<S, L>), cell_grid(g), dim3(kBlock), 0, st,
                        static_cast<typename S::T*>(dst), tipo8, g, p);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_pull_macros(Store sto, const void* src, const uint8_t* tipo8, float4* macros,
+                              const Geom& g, const StepParams& p, hipStream_t st) {
+  CFD_DISPATCH(sto, g, {
+    hipLaunchKernelGGL((k_pull_macros<S, L>), cell_grid(g), dim3(kBlock), 0, st,
+                       static_cast<const typename S::T*>(src), tipo8, macros, g, p);
   });
   return hipGetLastError();
 }

@@ -135,8 +135,11 @@ int cfd_get_front(cfd_solver *s, int *front);
 
 /* SolverWebGPU.passo (solver.js:395-415) device half: enqueue n steps of the
  * fused pull-stream + TRT + Smagorinsky + boundary kernel (wgsl.js:230-259),
- * flipping the front after each.  The macro field is written on the last of
- * the n steps (pulled, pre-collision moments; zero in solids).  Non-blocking. */
+ * flipping the front after each.  The macro field of the last step (its
+ * pulled, pre-collision moments; zero in solids, wgsl.js:255) is what
+ * cfd_read_macros returns; it is computed from the set that step read when
+ * it is first read (or before that set, the types or the layout change),
+ * not stored by every step.  Non-blocking. */
 int cfd_step(cfd_solver *s, int n);
 
 /* Launch tuning and population layout of the step kernel (autotuner hook;
