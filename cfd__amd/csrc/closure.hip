@@ -149,17 +149,18 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   const bool wall = is_wall(ct);
   if (!wall && (tb & NEAR_WALL)) bounce_back<false>(gi, sb, own, tipo, in, p, ct, nullptr);
 
-  const Moments M = moments(gi);
+  constexpr bool F = S::kFast;
+  const Moments M = moments<F>(gi);
   float pab[6];
   {
     // the equilibrium is recomputed after the MLP rather than held across it
     // (19 registers less at the matrix instructions: one more wave per SIMD)
     float e[Q];
-    equilibrium(M, e);
+    equilibrium<F>(M, e);
     noneq_stress(gi, e, pab);
   }
-  const float qs = stress_norm(pab);
-  const float tau_s = smagorinsky_tau(qs, M.rho, p);
+  const float qs = stress_norm<F>(pab);
+  const float tau_s = smagorinsky_tau<F>(qs, M.rho, p);
   const bool fluid = live && !wall && ct != ESPELHO_Y && ct != ESPELHO_Z && ct != ENTRADA &&
                      ct != SAIDA;
 
@@ -203,9 +204,9 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   float d[Q];
   if (!boundary_cell(ct, gi, M, c.z, p, d)) {
     float e[Q];
-    equilibrium(M, e);
-    trt(gi, e, tau, p, d);
-    sponge(c.x, p, d);
+    equilibrium<F>(M, e);
+    trt<F>(gi, e, tau, p, d);
+    sponge<F>(c.x, p, d);
   }
   // wall lanes of a wave that also holds other cells store zeros (never
   // read), so that every line of dst is written whole (no partial-line fetch)

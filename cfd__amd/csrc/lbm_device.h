@@ -44,8 +44,14 @@ __device__ __forceinline__ GLOBAL T* gptr(T* p) {
   return (GLOBAL T*)p;
 }
 
+// kFast: the per-cell arithmetic of the step may use single-instruction
+// reciprocals and square roots and explicit fused multiply-adds.  Off for
+// float32 storage, whose step rounds like the reference's shader text (and
+// the oracle) operation for operation; on for float16 storage, whose stored
+// values carry a 2^-11 rounding that dwarfs the few float32 ulps this moves.
 struct StoreF32 {
   static constexpr bool kStochastic = false;
+  static constexpr bool kFast = false;
   using T = float;
   using Bits = uint32_t;
   template <class P>
@@ -61,6 +67,7 @@ struct StoreF32 {
 // down to 1.5e-8 stay normal.  Collision still runs in float32.
 struct StoreF16 {
   static constexpr bool kStochastic = false;
+  static constexpr bool kFast = true;
   using T = _Float16;
   template <class P>
   static __device__ __forceinline__ float ld(P p) {
@@ -395,6 +402,7 @@ struct Moments {
   float delta, rho, ux, uy, uz, uu;
 };
 
+template <bool FAST = false>
 __device__ __forceinline__ Moments moments(const float (&gi)[Q]) {
   Moments M;
   float delta = gi[0];
@@ -420,20 +428,41 @@ __device__ __forceinline__ Moments moments(const float (&gi)[Q]) {
     }
     m[a] = acc;
   }
-  M.ux = m[0] / M.rho;
-  M.uy = m[1] / M.rho;
-  M.uz = m[2] / M.rho;
-  M.uu = M.ux * M.ux + M.uy * M.uy + M.uz * M.uz;
+  if constexpr (FAST) {
+    const float r = __builtin_amdgcn_rcpf(M.rho);
+    M.ux = m[0] * r;
+    M.uy = m[1] * r;
+    M.uz = m[2] * r;
+    M.uu = fmaf(M.ux, M.ux, fmaf(M.uy, M.uy, M.uz * M.uz));
+  } else {
+    M.ux = m[0] / M.rho;
+    M.uy = m[1] / M.rho;
+    M.uz = m[2] / M.rho;
+    M.uu = M.ux * M.ux + M.uy * M.uy + M.uz * M.uz;
+  }
   return M;
 }
 
-// Shifted equilibrium (ir.js:216-232).
+// Shifted equilibrium (ir.js:216-232).  FAST: the same polynomial as
+// w (delta + rho (cu (3 + 4.5 cu) - 1.5 uu)) with fused multiply-adds.
+template <bool FAST = false>
 __device__ __forceinline__ void equilibrium(const Moments& M, float (&e)[Q]) {
-  e[0] = wf(0) * (M.delta - 1.5f * M.rho * M.uu);
+  if constexpr (FAST) {
+    const float u15 = 1.5f * M.uu;
+    e[0] = wf(0) * fmaf(-M.rho, u15, M.delta);
 #pragma unroll
-  for (int i = 1; i < Q; ++i) {
-    const float cu = dotc(i, M.ux, M.uy, M.uz);
-    e[i] = wf(i) * (M.delta + M.rho * (3.0f * cu + 4.5f * cu * cu - 1.5f * M.uu));
+    for (int i = 1; i < Q; ++i) {
+      const float cu = dotc(i, M.ux, M.uy, M.uz);
+      const float q = fmaf(cu, fmaf(4.5f, cu, 3.0f), -u15);
+      e[i] = fmaf(wf(i) * M.rho, q, wf(i) * M.delta);
+    }
+  } else {
+    e[0] = wf(0) * (M.delta - 1.5f * M.rho * M.uu);
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const float cu = dotc(i, M.ux, M.uy, M.uz);
+      e[i] = wf(i) * (M.delta + M.rho * (3.0f * cu + 4.5f * cu * cu - 1.5f * M.uu));
+    }
   }
 }
 
@@ -466,21 +495,52 @@ __device__ __forceinline__ void noneq_stress(const float (&gi)[Q], const float (
 }
 
 // Q_s = sqrt(2 Pi:Pi) (ir.js:286-289).
+template <bool FAST = false>
 __device__ __forceinline__ float stress_norm(const float (&pab)[6]) {
-  const float pp = pab[0] * pab[0] + pab[1] * pab[1] + pab[2] * pab[2] +
-                   2.0f * (pab[3] * pab[3] + pab[4] * pab[4] + pab[5] * pab[5]);
-  return sqrtf(2.0f * pp);
+  if constexpr (FAST) {
+    const float dia = fmaf(pab[0], pab[0], fmaf(pab[1], pab[1], pab[2] * pab[2]));
+    const float off = fmaf(pab[3], pab[3], fmaf(pab[4], pab[4], pab[5] * pab[5]));
+    return __builtin_amdgcn_sqrtf(2.0f * fmaf(2.0f, off, dia));
+  } else {
+    const float pp = pab[0] * pab[0] + pab[1] * pab[1] + pab[2] * pab[2] +
+                     2.0f * (pab[3] * pab[3] + pab[4] * pab[4] + pab[5] * pab[5]);
+    return sqrtf(2.0f * pp);
+  }
 }
 
 // Closed-form Smagorinsky total relaxation time (ir.js:290-297).
+template <bool FAST = false>
 __device__ __forceinline__ float smagorinsky_tau(float qs, float rho, const StepParams& p) {
-  return 0.5f * (p.tau0 + sqrtf(p.tau0sq + p.c18cs2 * qs / rho));
+  if constexpr (FAST)
+    return 0.5f * (p.tau0 + __builtin_amdgcn_sqrtf(
+                               fmaf(p.c18cs2 * qs, __builtin_amdgcn_rcpf(rho), p.tau0sq)));
+  else
+    return 0.5f * (p.tau0 + sqrtf(p.tau0sq + p.c18cs2 * qs / rho));
 }
 
 // TRT collision per opposite pair with omega- from the fixed magic
 // parameter (ir.js:298-323).
+template <bool FAST = false>
 __device__ __forceinline__ void trt(const float (&gi)[Q], const float (&e)[Q], float tau,
                                     const StepParams& p, float (&d)[Q]) {
+  if constexpr (FAST) {
+    // omega+ = 1/tau; omega- = 1 / (Lambda / (tau - 1/2) + 1/2); per pair
+    // g -/+ (omega+/2) (sum_g - sum_e) -/+ (omega-/2) (dif_g - dif_e)
+    const float omP = __builtin_amdgcn_rcpf(tau);
+    const float omM = __builtin_amdgcn_rcpf(
+        fmaf(p.magic, __builtin_amdgcn_rcpf(tau - 0.5f), 0.5f));
+    const float hp = 0.5f * omP, hm = 0.5f * omM;
+    d[0] = fmaf(-omP, gi[0] - e[0], gi[0]);
+#pragma unroll
+    for (int i = 1; i < Q; i += 2) {
+      const int j = i + 1;
+      const float sp = (gi[i] + gi[j]) - (e[i] + e[j]);
+      const float sm = (gi[i] - gi[j]) - (e[i] - e[j]);
+      d[i] = fmaf(-hm, sm, fmaf(-hp, sp, gi[i]));
+      d[j] = fmaf(hm, sm, fmaf(-hp, sp, gi[j]));
+    }
+    return;
+  }
   const float omP = 1.0f / tau;
   const float omM = 1.0f / (p.magic / (1.0f / omP - 0.5f) + 0.5f);
   d[0] = gi[0] - omP * (gi[0] - e[0]);
@@ -498,13 +558,15 @@ __device__ __forceinline__ void trt(const float (&gi)[Q], const float (&e)[Q], f
 // and mix returns dst unchanged while teq is finite, so the blend is skipped
 // there; a non-finite free stream (sponge_all, uniform) blends every fluid
 // cell as the reference's mix does.
+template <bool FAST = false>
 __device__ __forceinline__ void sponge(uint32_t x, const StepParams& p, float (&d)[Q]) {
   const float dxs = (float)x - p.sponge_start;
   if (dxs > 0.0f || p.sponge_all) {
     const float sigma = fminf(fmaxf(dxs / p.sponge_len_m, 0.0f), 1.0f);
     const float sig2 = sigma * sigma;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) d[i] = d[i] * (1.0f - sig2) + p.tinf[i] * sig2;
+    for (int i = 0; i < Q; ++i)
+      d[i] = FAST ? fmaf(sig2, p.tinf[i] - d[i], d[i]) : d[i] * (1.0f - sig2) + p.tinf[i] * sig2;
   }
 }
 

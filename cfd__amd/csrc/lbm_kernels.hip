@@ -129,7 +129,8 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
 
   // 2. moments, then the branch chain ESPELHO_Y -> ESPELHO_Z -> ENTRADA ->
   //    SAIDA -> fluid (equilibrium, Smagorinsky, TRT, sponge).
-  const Moments M = moments(gi);
+  constexpr bool F = S::kFast;
+  const Moments M = moments<F>(gi);
   if (PULL_ONLY) {
     macros[cell] = make_float4(M.ux, M.uy, M.uz, M.delta);
     return;
@@ -137,11 +138,11 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
   float d[Q];
   if (!boundary_cell(ct, gi, M, c.z, p, d)) {
     float e[Q];
-    equilibrium(M, e);
+    equilibrium<F>(M, e);
     float pab[6];
     noneq_stress(gi, e, pab);
-    trt(gi, e, smagorinsky_tau(stress_norm(pab), M.rho, p), p, d);
-    sponge(c.x, p, d);
+    trt<F>(gi, e, smagorinsky_tau<F>(stress_norm<F>(pab), M.rho, p), p, d);
+    sponge<F>(c.x, p, d);
   }
 
   const uint32_t h = S::kStochastic ? sr_hash(cell, p.sr_seed) : 0u;

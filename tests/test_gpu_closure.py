@@ -317,3 +317,76 @@ def test_wall_distance_follows_the_types_unless_given():
         assert np.array_equal(s.lerDistanciaParede(), ref(tipo_b))
     finally:
         s.destruir()
+
+
+def test_c5_closure_at_size(orc):
+    """The learned-closure workload itself: the shipped model on the coarse
+    240x120x96 vehicle tunnel (seed 7 at 0.75 scale, no-slip floor, inlet
+    boundary layer, sponge; omega 1.90, Cs 0.1).  From a flow developed by
+    1500 plain steps, two closure steps on the GPU agree with the oracle's
+    closure step (float32: 1e-5 after one step; after two, the 99th
+    percentile, with the bf16 rounding flips of the matrix cores bounded
+    at 3e-4), with the wall distance the device computed; the MLP is in use
+    in at least 90 % of the fluid cell-steps.  Then the float16-storage form
+    with stochastic rounding that the workload runs: 600 closure steps stay
+    finite with the same fallback bound."""
+    from cfd__amd import fechamento_caso as fc
+    from cfd__amd.fechamento import Fechamento
+    from cfd__amd.solver import SolverMI355X
+    caso = fc.montar(fc.AVALIACAO_SEED, 1)
+    g, tipo = caso["grade"], caso["tipo"]
+    grid = (g["nx"], g["ny"], g["nz"])
+    fech = Fechamento.carregar(fc.MODELO)
+    cfg = dict(omegaPlus=fc.omega_de(fc.NU_C), lesCs=fc.CS, esponja=caso["esponja"],
+               inletU=[fc.U, 0.0, 0.0], beltU=[0.0, 0.0, 0.0],
+               espessuraCL=float(np.floor(g["nz"] * 0.12 + 0.5)))
+    wall = np.isin(tipo, [1, 4, 7])
+    fluid = int(np.count_nonzero(tipo == 0))
+    s = SolverMI355X.criar(g)
+    try:
+        s.definirTipos(tipo)
+        s.configurar(**cfg)
+        s.inicializar()
+        s.passo(1500)
+        g0 = s.lerPopulacoes()
+        s.calcularDistanciaParede()
+        wdq = s.lerDistanciaParede().astype(np.float32) / np.float32(8)
+        s.definirFechamento(fech)
+        s.fallbacksFechamento()
+        got = []
+        for _ in range(2):
+            s.passo(1)
+            got.append(s.lerPopulacoes())
+        fb = s.fallbacksFechamento()
+    finally:
+        s.destruir()
+    assert fb <= 0.1 * 2 * fluid, fb / (2 * fluid)
+    par = orc.params_array(nx=grid[0], **cfg)
+    cl = orc.closure_array(fech.params, fech.shift, fech.scale, fech.lo, fech.hi, fech.out_scale)
+    ref = g0.reshape(-1).copy()
+    for k in range(2):
+        ref, _ = orc.step_closure(grid, ref, tipo, par, cl, wdq)
+        r = ref.reshape(19, -1)[:, ~wall]
+        err = np.abs(got[k][:, ~wall].astype(np.float64) - r).max(axis=0) / np.abs(r).max()
+        print(f"closure step {k + 1}: max {err.max():.2e}, p99 {np.percentile(err, 99):.2e}",
+              flush=True)
+        if k == 0:
+            assert err.max() <= 1e-5, err.max()
+        else:
+            assert err.max() <= 3e-4 and np.percentile(err, 99) <= 1e-5, (
+                err.max(), np.percentile(err, 99))
+
+    s = SolverMI355X.criar(g, precisao="fp16sr")
+    try:
+        s.definirTipos(tipo)
+        s.configurar(**cfg)
+        s.inicializar()
+        s.calcularDistanciaParede()
+        s.definirFechamento(fech)
+        s.passo(600)
+        fb = s.fallbacksFechamento()
+        _, bad = s.massa()
+    finally:
+        s.destruir()
+    assert bad == 0
+    assert fb <= 0.1 * 600 * fluid, fb / (600 * fluid)
