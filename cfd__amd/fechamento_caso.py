@@ -83,10 +83,12 @@ def _units():
 
 def rodar(caso: dict, passos: int, precisao: str = "fp32", fech: Fechamento | None = None,
           amostra: int = 100, media_desde: int | None = None, instantaneos=(), ao_instantaneo=None,
-          log=print) -> dict:
+          campo_cada: int = 0, log=print) -> dict:
     """One tunnel run: ramp over 1000 coarse-equivalent steps, then `passos`
     steps; forces every `amostra` steps; Cd / Cl / St over steps >=
-    media_desde; `ao_instantaneo(solver, k)` at the listed steps."""
+    media_desde; `ao_instantaneo(solver, k)` at the listed steps; with
+    campo_cada > 0 also the time-averaged velocity field over the same window,
+    sampled every campo_cada steps ('campo', float32 (nz, ny, nx, 3))."""
     from .solver import SolverMI355X
 
     g, fator = caso["grade"], caso["fator"]
@@ -106,7 +108,11 @@ def rodar(caso: dict, passos: int, precisao: str = "fp32", fech: Fechamento | No
         media_desde = passos // 2 if media_desde is None else media_desde
         cd, cl = JanelaCd(10 ** 7), JanelaCd(10 ** 7)
         serie_cl, serie_cs = [], []
-        pendentes = sorted(k for k in set(instantaneos) if 0 < k <= passos)
+        marcas = set(instantaneos)
+        if campo_cada > 0:
+            marcas |= set(range(media_desde + campo_cada, passos + 1, campo_cada))
+        pendentes = sorted(k for k in marcas if 0 < k <= passos)
+        soma_u, n_campo = None, 0
         kernel_ms = 0.0
         k = 0
         while k < passos:
@@ -116,7 +122,12 @@ def rodar(caso: dict, passos: int, precisao: str = "fp32", fech: Fechamento | No
             kernel_ms += s.passoCronometrado(n)
             k += n
             if pendentes and k == pendentes[0]:
-                ao_instantaneo(s, k)
+                if ao_instantaneo is not None and k in instantaneos:
+                    ao_instantaneo(s, k)
+                if campo_cada > 0 and k > media_desde and (k - media_desde) % campo_cada == 0:
+                    m = s.lerMacros().reshape(-1, 4)[:, :3]
+                    soma_u = m.astype(np.float64) if soma_u is None else soma_u + m
+                    n_campo += 1
                 pendentes.pop(0)
             if k % amostra == 0 and k >= media_desde:
                 co = coeficientes(s.medirForcas(), _units(), caso["area"], g,
@@ -142,7 +153,20 @@ def rodar(caso: dict, passos: int, precisao: str = "fp32", fech: Fechamento | No
             "st_comprimento": st, "amostras": cd.n, "kernel_ms": kernel_ms,
             "mlups": N * passos / (kernel_ms * 1e-3) / 1e6, "nao_finitos": nao_finitos,
             "fallback_frac": fallbacks / max(1, fluid * passos) if fech is not None else None,
-            "passos": passos, "precisao": precisao, "fechamento": fech is not None}
+            "passos": passos, "precisao": precisao, "fechamento": fech is not None,
+            "campo": (soma_u / n_campo).astype(np.float32).reshape(g["nz"], g["ny"], g["nx"], 3)
+            if n_campo else None, "campo_amostras": n_campo}
+
+
+def erro_esteira(campo_grosso: np.ndarray, campo_fino: np.ndarray, grosso: dict) -> float:
+    """Solver-in-the-loop error of a coarse run: RMS over the coarse fluid
+    cells of |<u>_coarse - box2(<u>_fine)| / U, both time averages over the
+    same physical window (the mean of the box-filtered fine field is the box
+    filter of its mean)."""
+    ref = _box2(campo_fino.astype(np.float64))
+    fl = (np.asarray(grosso["tipo"]).reshape(campo_grosso.shape[:3]) == 0)
+    d = (campo_grosso.astype(np.float64) - ref)[fl]
+    return float(np.sqrt((d * d).sum(axis=-1).mean()) / U)
 
 
 # ------------------------------------------------------------------ data
@@ -246,6 +270,43 @@ def gerar_dados(seeds=TREINO_SEEDS, passos_finos: int = 12000, instantaneos: int
 
 # ------------------------------------------------------------------ calibration
 
+def calibrar_esteira(fech: Fechamento, seeds=TREINO_SEEDS[:3],
+                     fatores=(0.25, 0.5, 1.0, 2.0, 4.0, 8.0), passos_grossos: int = 9000,
+                     log=print) -> dict:
+    """Solver in the loop, a posteriori: scale the trained output by each
+    factor, run the coarse float16 (stochastic rounding) tunnel of each
+    calibration shape and keep the factor whose time-averaged velocity field
+    is closest to the box-filtered fine one (erro_esteira, mean over the
+    shapes).  The field error averages over every fluid cell and the whole
+    window, so it moves far less between runs than the windowed Cd of an
+    unsteady coarse wake; the a-priori fit only sees the filtered sub-grid
+    stress, this sees what the coarse run does with it."""
+    finos, grossos = {}, {}
+    desde = passos_grossos // 3
+    for sd in seeds:
+        grossos[sd] = montar(sd, 1)
+        finos[sd] = rodar(montar(sd, 2), 2 * passos_grossos, "fp32", amostra=200,
+                          media_desde=2 * desde, campo_cada=2 * 300)["campo"]
+        log(f"[calibrar] fine field of shape {sd}")
+    medidas = []
+    base = fech.out_scale
+    for fa in fatores:
+        fech.out_scale = base * fa
+        errs = [erro_esteira(rodar(grossos[sd], passos_grossos, "fp16sr", fech=fech,
+                                   media_desde=desde, campo_cada=300)["campo"], finos[sd],
+                             grossos[sd]) for sd in seeds]
+        medidas.append({"fator": fa, "erro_esteira": errs, "erro_medio": float(np.mean(errs))})
+        log(f"[calibrar] factor {fa:g}: mean wake-field error {np.mean(errs):.5f}")
+    smag = [erro_esteira(rodar(grossos[sd], passos_grossos, "fp16sr", media_desde=desde,
+                               campo_cada=300)["campo"], finos[sd], grossos[sd]) for sd in seeds]
+    melhor = min(medidas, key=lambda m: m["erro_medio"])
+    fech.out_scale = base * melhor["fator"]
+    log(f"[calibrar] chosen factor {melhor['fator']:g} (Smagorinsky: {np.mean(smag):.5f})")
+    return {"medidas": medidas, "smagorinsky": {"erro_esteira": smag,
+                                                "erro_medio": float(np.mean(smag))},
+            "fator": melhor["fator"], "out_scale": fech.out_scale}
+
+
 def calibrar(fech: Fechamento, seeds=TREINO_SEEDS[:3], fatores=(1.0, 10.0, 100.0, 1000.0),
              passos_grossos: int = 9000, log=print) -> dict:
     """Solver in the loop: scale the trained output by each factor, run the
@@ -284,16 +345,20 @@ def avaliar(fech: Fechamento, seed: int = AVALIACAO_SEED, passos_grossos: int = 
             log=print) -> dict:
     fino, grosso = montar(seed, 2), montar(seed, 1)
     out = {}
+    desde = passos_grossos // 3
     out["fino_fp32_smagorinsky"] = rodar(fino, 2 * passos_grossos, "fp32", amostra=200,
-                                         media_desde=2 * passos_grossos // 3)
+                                         media_desde=2 * desde, campo_cada=2 * 300)
     log(f"[avaliar] fine Cd {out['fino_fp32_smagorinsky']['cd']:.4f}")
     for nome, prec, f in (("grosso_fp32_smagorinsky", "fp32", None),
                           ("grosso_fp16sr_smagorinsky", "fp16sr", None),
                           ("grosso_fp16sr_aprendido", "fp16sr", fech),
                           ("grosso_fp32_aprendido", "fp32", fech)):
         out[nome] = rodar(grosso, passos_grossos, prec, fech=f, amostra=100,
-                          media_desde=passos_grossos // 3)
-        log(f"[avaliar] {nome}: Cd {out[nome]['cd']:.4f}, {out[nome]['mlups']:.0f} MLUPS")
+                          media_desde=desde, campo_cada=300)
+        out[nome]["erro_esteira"] = erro_esteira(out[nome]["campo"],
+                                                 out["fino_fp32_smagorinsky"]["campo"], grosso)
+        log(f"[avaliar] {nome}: Cd {out[nome]['cd']:.4f}, wake-field error "
+            f"{out[nome]['erro_esteira']:.5f}, {out[nome]['mlups']:.0f} MLUPS")
     cd_f = out["fino_fp32_smagorinsky"]["cd"]
     st_f = out["fino_fp32_smagorinsky"]["st_comprimento"]["lateral"]
     err = {k: (v["cd"] - cd_f) / cd_f for k, v in out.items() if k.startswith("grosso")}
@@ -308,7 +373,11 @@ def avaliar(fech: Fechamento, seed: int = AVALIACAO_SEED, passos_grossos: int = 
         "mlups_equivalente_fino": 16.0 * out["grosso_fp16sr_aprendido"]["mlups"],
         "ganho_vs_fino": ganho,
         "juiz": "aprendido" if aceito else "smagorinsky (o aprendido ficou mais longe do fino)",
+        "erro_esteira": {k: v["erro_esteira"] for k, v in out.items() if k.startswith("grosso")},
     }
+    for v in out.values():
+        if isinstance(v, dict):
+            v.pop("campo", None)
     return out
 
 
@@ -322,6 +391,9 @@ def main(argv=None):
     ap.add_argument("--modelo", default=None, help="where to save the trained closure (.npz)")
     ap.add_argument("--calibrar", action="store_true",
                     help="choose the output scale by coarse-vs-fine Cd on training shapes")
+    ap.add_argument("--calibrar-esteira", action="store_true",
+                    help="choose the output scale by the coarse-vs-fine time-averaged velocity "
+                         "field on training shapes (solver in the loop), before judging")
     ap.add_argument("--perfil", action="store_true",
                     help="only the coarse float16 (stochastic rounding) closure run (rocprofv3)")
     ap.add_argument("--carregar", default=None,
@@ -344,6 +416,10 @@ def main(argv=None):
         sementes = [int(x) for x in a.sementes.split(",")]
         assert not set(sementes) & set(TREINO_SEEDS), "judging seeds must be held out"
         res = {"por_semente": {}}
+        if a.calibrar_esteira:
+            res["calibracao_esteira"] = calibrar_esteira(fech, passos_grossos=a.passos_grossos)
+            if a.modelo:
+                fech.salvar(a.modelo)
         for sd in sementes:
             r = avaliar(fech, seed=sd, passos_grossos=a.passos_grossos)
             res["por_semente"][sd] = r["resumo"]
@@ -357,7 +433,9 @@ def main(argv=None):
                 "erro_medio": float(np.mean([r["erro_cd"][v]
                                              for r in res["por_semente"].values()])),
                 "dentro_3pct": int(sum(abs(r["erro_cd"][v]) <= 0.03
-                                       for r in res["por_semente"].values()))}
+                                       for r in res["por_semente"].values())),
+                "erro_esteira_medio": float(np.mean([r["erro_esteira"][v]
+                                                     for r in res["por_semente"].values()]))}
             for v in variantes}
         res["resumo"]["sementes"] = sementes
         if a.saida:

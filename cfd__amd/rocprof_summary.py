@@ -72,6 +72,9 @@ def main():
     ap.add_argument("--out", type=Path, required=True)
     ap.add_argument("--last", type=int, default=0,
                     help="average only the last N dispatches of the main instantiation")
+    ap.add_argument("--match", default="",
+                    help="the main instantiation's name must also contain this text "
+                         "(e.g. 'StoreF32, false, 512')")
     a = ap.parse_args()
 
     st = _stats(a.trace)
@@ -81,7 +84,7 @@ def main():
     kib = a.copy_bytes / 1024
     ff, fw = kib / cf, kib / cw
     # the step kernel without the macro store is the one timed per step
-    step_rows = [r for r in st if a.kernel in r[0]]
+    step_rows = [r for r in st if a.kernel in r[0] and a.match in r[0]]
     main_row = max(step_rows, key=lambda r: r[1])
     kname = main_row[0]
     # the counter entry of the same instantiation as the trace's main row
@@ -89,7 +92,8 @@ def main():
     def same(dct):
         best = None
         for k, v in dct.items():
-            if a.kernel in k and ("true" in k) == ("true" in kname):
+            if k == kname or (best is None and a.kernel in k and a.match in k and
+                              ("true" in k) == ("true" in kname)):
                 best = v
         return best
     fk, wk = same(fetch), same(write)
@@ -114,7 +118,8 @@ def main():
         f"--kernel {a.kernel} --cells {a.cells} --steps {a.steps} "
         f"--bytes-per-cell {a.bytes_per_cell} "
         f"--copy-bytes {a.copy_bytes} --command '...' --out {a.out}"
-        + (f" --last {a.last}" if a.last > 0 else ""),
+        + (f" --last {a.last}" if a.last > 0 else "")
+        + (f" --match '{a.match}'" if a.match else ""),
         "```",
         "",
         "## Kernel trace (--stats)",
