@@ -5,9 +5,8 @@ content (node TRS and matrix transforms, a child node, interleaved vertex
 data, uint8 / uint16 / uint32 indices, a primitive without indices, a LINES
 primitive that must be skipped, a Draco primitive that must be skipped), and a
 reduced synthetic vehicle laid with flow along y and up along x, as GLB and as
-binary STL.  The expected values are closed forms computed in the tests (the
-reference's parse.js / prepare.js use syntax that the Node 12 of this image
-cannot evaluate, so they are not run).
+binary STL.  The tests check them against closed forms, and against the
+reference's own parse.js / prepare.js through make_prepare_golden.py.
 
 usage: python tests/golden/make_mesh_golden.py
 """

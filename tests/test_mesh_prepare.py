@@ -188,3 +188,49 @@ def test_geometry_page(orc, nome):
     af = tunel.areaFrontal(v["tipo"], g)
     assert s["nSolido"] > 200 and s["fracaoSolida"] < 0.35 and af > 50, (nome, s, af)
     assert not s["ocoSuspeito"], (nome, s)
+
+
+# ---------------------------------------------------------------- reference placement
+
+GOLDEN_PREP = Path(__file__).resolve().parent / "golden" / "prepare_golden.json"
+
+
+def _prep_cases(tmp):
+    import sys
+    sys.path.insert(0, str(GOLDEN_PREP.parent))
+    from prepare_cases import cases
+    return {c["id"]: c for c in cases(tmp)}
+
+
+def _sha(a, dtype):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=dtype).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("cid", ["cube_baixa", "vehicle_stl_baixa", "vehicle_glb_baixa",
+                                 "vehicle_glb_yaw_flip", "nodes_glb_default",
+                                 "nodes_gltf_default", "vehicle200k_media_32"])
+def test_placement_equals_the_reference(cid, tmp_path):
+    """ler() and preparar() against the reference's own parse.js / prepare.js
+    run on the same files (tests/golden/prepare_golden.json, written by
+    tests/golden/make_prepare_golden.py): the read positions and indices and
+    the placed float32 positions bit for bit, and the placement's guesses and
+    scalars (axis order, direction, unit factor, scale, blockage, extents)."""
+    import json
+    ref = json.loads(GOLDEN_PREP.read_text())["cases"][cid]
+    c = _prep_cases(tmp_path)[cid]
+    m = mesh_io.ler(Path(c["file"]).read_bytes(), c["name"])
+    assert _sha(m["positions"], np.float32) == ref["read_positions_sha256"], cid
+    assert _sha(m["indices"], np.uint32) == ref["read_indices_sha256"], cid
+    p = preparo.preparar(m, c["grade"], c["opcoes"])
+    assert list(p["eixos"]["ordem"]) == ref["ordem"], cid
+    assert bool(p["sentido"]["aplicado"]) == ref["inverter"], cid
+    assert p["sentido"]["alturaFrente"] == ref["alturaFrente"], cid
+    assert p["sentido"]["alturaTras"] == ref["alturaTras"], cid
+    assert p["unidade"]["fator"] == ref["fator"], cid
+    assert p["escala"] == ref["escala"], cid
+    assert p["comprimentoM"] == ref["comprimentoM"], cid
+    assert p["bloqueioCaixa"] == ref["bloqueioCaixa"], cid
+    assert list(p["extentos"]["min"]) == ref["min"] and list(p["extentos"]["max"]) == ref["max"]
+    assert np.asarray(p["positions"]).size == ref["n_positions"], cid
+    assert _sha(p["positions"], np.float32) == ref["positions_sha256"], cid
