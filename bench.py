@@ -44,7 +44,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0          # MI355X published HBM3E bandwidth
-BYTES_PER_CELL = {"fp32": 152, "fp16": 76, "fp16sr": 76}   # 19 pops read + 19 written
+BYTES_PER_CELL = {"fp32": 152, "fp32exact": 152, "fp16": 76, "fp16sr": 76}  # 19 read + 19 written
 GRID = (320, 160, 128)
 U_IN = 0.08
 RE_REQUESTED = 1.0e5
@@ -62,7 +62,8 @@ def parse_args():
     ap.add_argument("--ajuste", default=None,
                     help='fixed launch order as JSON, e.g. \'{"sweep": 0, "rotate": 9216}\' '
                          "(skips the launch-order autotuner)")
-    ap.add_argument("--precision", choices=("fp32", "fp16", "fp16sr"), default="fp32")
+    ap.add_argument("--precision", choices=("fp32", "fp32exact", "fp16", "fp16sr"),
+                    default="fp32")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="budget of the CPU-baseline sample (0 disables it)")
     ap.add_argument("--workload", choices=("vehicle", "sphere"), default="vehicle",
@@ -381,7 +382,7 @@ def main_sphere(args):
         "value": round(cells * args.steps / wall / 1e6, 1), "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if args.precision == "fp32" else "f16-storage/f32-collide",
+        "dtype": "f32" if args.precision.startswith("fp32") else "f16-storage/f32-collide",
         "data": f"synthetic (seeded {c['n_tri']}-triangle icosphere, GPU-voxelized)",
         "config": {"workload": f"plain D3Q19 TRT(3/16)+Smagorinsky(0.1) {args.precision} "
                                "sphere D16 160x80x64, u_in=0.05, Re=100",
@@ -434,7 +435,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if precision == "fp32" else "f16-storage/f32-collide",
+        "dtype": "f32" if precision.startswith("fp32") else "f16-storage/f32-collide",
         "data": "synthetic (seeded 200k-triangle vehicle, GPU-voxelized)",
         "config": {
             "workload": f"plain D3Q19 TRT(3/16)+Smagorinsky(0.1) {precision} vehicle "

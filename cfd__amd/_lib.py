@@ -18,6 +18,7 @@ ABI_VERSION = 3
 CFD_STORE_FP32 = 0
 CFD_STORE_FP16 = 1
 CFD_ROUND_STOCHASTIC = 2
+CFD_ROUND_EXACT = 4
 
 
 class CFDError(RuntimeError):

@@ -125,7 +125,8 @@ float dotc_host(int i, const float v[3]) {
   return acc;
 }
 
-size_t elem_bytes(const cfd_solver* s) { return s->store == Store::F32 ? 4 : 2; }
+bool is_f32(Store s) { return s == Store::F32 || s == Store::F32X; }
+size_t elem_bytes(const cfd_solver* s) { return is_f32(s->store) ? 4 : 2; }
 
 // 36 % of the step's launch grid (workgroups, padded to whole XCD chunk
 // spans as launch_step pads it).
@@ -216,7 +217,8 @@ int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
     return fail(CFD_E_ARG, "cfd_create: %llu cells exceed the 2^30 limit of one handle",
                 (unsigned long long)N);
   if (flags != CFD_STORE_FP32 && flags != CFD_STORE_FP16 &&
-      flags != (CFD_STORE_FP16 | CFD_ROUND_STOCHASTIC))
+      flags != (CFD_STORE_FP16 | CFD_ROUND_STOCHASTIC) &&
+      flags != (CFD_STORE_FP32 | CFD_ROUND_EXACT))
     return fail(CFD_E_ARG, "cfd_create: unknown storage flags 0x%x", flags);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -227,9 +229,11 @@ int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
   s->ny = ny;
   s->nz = nz;
   s->N = (size_t)N;
-  s->store = flags == CFD_STORE_FP32 ? Store::F32
-                                     : (flags & CFD_ROUND_STOCHASTIC ? Store::F16S : Store::F16);
-  const size_t eb = s->store == Store::F32 ? 4 : 2;
+  s->store = flags == CFD_STORE_FP32                        ? Store::F32
+             : flags == (CFD_STORE_FP32 | CFD_ROUND_EXACT) ? Store::F32X
+             : (flags & CFD_ROUND_STOCHASTIC)              ? Store::F16S
+                                                           : Store::F16;
+  const size_t eb = is_f32(s->store) ? 4 : 2;
   s->geom = make_geom(nx, ny, nz, s->tuning.layout, (uint32_t)s->tuning.slice_pad);
   if (s->geom.pop_elems * eb >= (1ull << 32)) {
     // the blocked layout addresses a set with 32-bit byte offsets
@@ -244,14 +248,14 @@ int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver** out) {
   // arithmetic and gained at every size measured.  A float32 set larger than
   // the cache lost 1.5-2 % with it, one of 62 MB about 1 %.
   s->tuning.sweep = s->pop_bytes > (128ull << 20) &&
-                    (s->store != Store::F32 || s->pop_bytes <= (256ull << 20));
+                    (!is_f32(s->store) || s->pop_bytes <= (256ull << 20));
   // A float32 set larger than the cache (the 6.55 M-cell vehicle's 498 MB):
   // 512-thread workgroups with the start rotated by 36 % of the launch grid
   // per step, the launch-order autotuner's pick on every box measured
   // (174.3-175.1 us per step on the vehicle against 175.5-177.5 us for
   // 256-thread workgroups in ascending order).  Every shape gives the same
   // bits, so this only sets what a run that does not autotune gets.
-  if (s->store == Store::F32 && s->pop_bytes > (256ull << 20) &&
+  if (is_f32(s->store) && s->pop_bytes > (256ull << 20) &&
       s->tuning.layout == kLayoutBlocked) {
     s->tuning.block = 512;
     s->tuning.rotate = default_rotation(s->N, 512, s->tuning.remap);

@@ -66,8 +66,9 @@ struct Tuning {
 };
 
 // Population storage: float32 (152 B/cell/step) or scaled float16 (76 B),
-// the latter rounded to nearest or stochastically by the step.
-enum class Store { F32 = 0, F16 = 1, F16S = 2 };
+// the latter rounded to nearest or stochastically by the step; F32X is
+// float32 with the step's arithmetic rounded exactly as the oracle's.
+enum class Store { F32 = 0, F16 = 1, F16S = 2, F32X = 3 };
 
 hipError_t launch_prep_types(const uint32_t* tipo32, uint8_t* tipo8, const Geom& g,
                              hipStream_t st);

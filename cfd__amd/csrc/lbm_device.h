@@ -44,14 +44,17 @@ __device__ __forceinline__ GLOBAL T* gptr(T* p) {
   return (GLOBAL T*)p;
 }
 
-// kFast: the per-cell arithmetic of the step may use single-instruction
-// reciprocals and square roots and explicit fused multiply-adds.  Off for
-// float32 storage, whose step rounds like the reference's shader text (and
-// the oracle) operation for operation; on for float16 storage, whose stored
-// values carry a 2^-11 rounding that dwarfs the few float32 ulps this moves.
+// kFast: the per-cell arithmetic of the step uses single-instruction
+// reciprocals and square roots (1 ulp) and explicit fused multiply-adds: a
+// quotient within ~2 ulp, inside the 2.5 ulp WGSL allows for the division of
+// the reference's shader, and 8 correctly rounded divisions fewer per cell.
+// On for float32 and float16 storage; StoreF32X (CFD_ROUND_EXACT) turns it
+// off and rounds like the reference's shader text read with correctly
+// rounded division and square root, operation for operation (the oracle's
+// arithmetic, bit for bit).
 struct StoreF32 {
   static constexpr bool kStochastic = false;
-  static constexpr bool kFast = false;
+  static constexpr bool kFast = true;
   using T = float;
   using Bits = uint32_t;
   template <class P>
@@ -60,6 +63,10 @@ struct StoreF32 {
   static __device__ __forceinline__ void st(P p, float v) { *p = v; }
   static __device__ __forceinline__ float dec(Bits b) { return __uint_as_float(b); }
   static __device__ __forceinline__ Bits enc(float v) { return __float_as_uint(v); }
+};
+
+struct StoreF32X : StoreF32 {
+  static constexpr bool kFast = false;
 };
 
 // float16 deviations g_i = f_i - w_i scaled by 2^12 (exact power of two, so
@@ -619,6 +626,10 @@ __device__ __forceinline__ bool boundary_cell(uint32_t ct, const float (&gi)[Q],
   do {                                                                     \
     if ((sto) == Store::F32) {                                             \
       using S = StoreF32;                                                  \
+      if ((g).lb) { constexpr int L = kBlockLog2; __VA_ARGS__ }            \
+      else { constexpr int L = 0; __VA_ARGS__ }                            \
+    } else if ((sto) == Store::F32X) {                                     \
+      using S = StoreF32X;                                                 \
       if ((g).lb) { constexpr int L = kBlockLog2; __VA_ARGS__ }            \
       else { constexpr int L = 0; __VA_ARGS__ }                            \
     } else if ((sto) == Store::F16) {                                      \

@@ -33,11 +33,15 @@ class SolverMI355X {
   /** The GPU context is process-wide; kept for API parity (solver.js:61-82). */
   static async criarDevice() { return { tipo: 'mi355x', versao: addon.version() }; }
 
-  /** Allocate a solver on the GPU (solver.js:93-124).  precisao 'fp32' | 'fp16'. */
+  /** Allocate a solver on the GPU (solver.js:93-124).  precisao 'fp32' |
+   *  'fp32exact' (every operation rounded as the CPU restatement rounds it) |
+   *  'fp16' | 'fp16sr'. */
   static async criar(ctx, grade, precisao) {
     const p = precisao || 'fp32';
-    const flags = { fp32: 0, fp16: 1, fp16sr: 3 }[p];
-    if (flags === undefined) throw new Error(`precisao deve ser fp32, fp16 ou fp16sr: ${p}`);
+    const flags = { fp32: 0, fp32exact: 4, fp16: 1, fp16sr: 3 }[p];
+    if (flags === undefined) {
+      throw new Error(`precisao deve ser fp32, fp32exact, fp16 ou fp16sr: ${p}`);
+    }
     const h = addon.create(grade.nx, grade.ny, grade.nz, flags);
     return new SolverMI355X(h, grade, p);
   }

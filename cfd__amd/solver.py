@@ -36,14 +36,18 @@ class SolverMI355X:
     def criar(grade, precisao: str = "fp32") -> "SolverMI355X":
         """Allocate a solver for grade = {nx, ny, nz} (solver.js:93-124).
 
-        precisao: 'fp32' (152 B/cell/step), 'fp16' (76 B/cell/step, rounded to
-        nearest) or 'fp16sr' (76 B/cell/step, stochastic rounding)."""
+        precisao: 'fp32' (152 B/cell/step), 'fp32exact' (the same with every
+        operation rounded as the oracle rounds it: correctly rounded division
+        and square root), 'fp16' (76 B/cell/step, rounded to nearest) or
+        'fp16sr' (76 B/cell/step, stochastic rounding)."""
         nx, ny, nz = (int(grade["nx"]), int(grade["ny"]), int(grade["nz"])) \
             if isinstance(grade, dict) else (int(grade[0]), int(grade[1]), int(grade[2]))
         flags = {"fp32": CFD_STORE_FP32, "fp16": CFD_STORE_FP16,
+                 "fp32exact": CFD_STORE_FP32 | _lib.CFD_ROUND_EXACT,
                  "fp16sr": CFD_STORE_FP16 | _lib.CFD_ROUND_STOCHASTIC}.get(precisao)
         if flags is None:
-            raise ValueError(f"precisao must be 'fp32', 'fp16' or 'fp16sr', not {precisao!r}")
+            raise ValueError("precisao must be 'fp32', 'fp32exact', 'fp16' or 'fp16sr', "
+                             f"not {precisao!r}")
         h = C.c_void_p()
         check(lib.cfd_create(nx, ny, nz, flags, C.byref(h)))
         return SolverMI355X(h, nx, ny, nz, precisao)

@@ -36,6 +36,11 @@ extern "C" {
 #define CFD_STORE_FP16 1u   /* populations stored as scaled float16 deviations */
 #define CFD_ROUND_STOCHASTIC 2u /* with FP16: the step rounds each stored value up or
                                    down at random in proportion (unbiased)      */
+#define CFD_ROUND_EXACT 4u   /* with FP32: the step rounds every operation as the
+                                reference's shader text with correctly rounded
+                                division and square root (default: 1-ulp
+                                reciprocal / square root and fused multiply-adds,
+                                within WGSL's 2.5 ulp division accuracy)        */
 
 /* Cell types, identical to CELULA in src/core/emit/ir.js:71-80. */
 #define CFD_FLUIDO 0u
@@ -103,7 +108,7 @@ typedef struct cfd_vox_stats {
  * population sets (A and B), the cell-type field and the macro field, all
  * zero-filled: g_i = 0 is fluid at rest with rho = 1 exactly (lattice.js:32-34).
  * The front starts at A.  flags: CFD_STORE_FP32, CFD_STORE_FP16 or
- * CFD_STORE_FP16 | CFD_ROUND_STOCHASTIC. */
+ * CFD_STORE_FP16 | CFD_ROUND_STOCHASTIC or CFD_STORE_FP32 | CFD_ROUND_EXACT. */
 int cfd_create(int nx, int ny, int nz, uint32_t flags, cfd_solver **out);
 
 /* SolverWebGPU.configurar (solver.js:277-293). */

@@ -185,7 +185,12 @@ def test_closure_with_zero_viscosity_is_the_cs0_step_bit_for_bit():
     assert np.array_equal(a[:, ~wall], b[:, ~wall])
 
 
-def test_closure_step_matches_oracle(orc):
+@pytest.mark.parametrize("precisao", ["fp32exact", "fp32"])
+def test_closure_step_matches_oracle(orc, precisao):
+    """With the exact float32 rounding the features equal the oracle's bit for
+    bit and only the matrix cores' summation order differs; with the default
+    1-ulp arithmetic a feature can also round to the other bf16 neighbour
+    after one step, so both steps take the percentile form of the bound."""
     from cfd__amd.fechamento import distancia_parede
     g, tipo, esp = _tunnel()
     grid = (g["nx"], g["ny"], g["nz"])
@@ -203,7 +208,7 @@ def test_closure_step_matches_oracle(orc):
         for _ in range(target - done):
             ref, _ = orc.step_closure(grid, ref, tipo, par, cl, wdq)
         done = target
-        got, fb = _run(g, tipo, esp, g0, target, fech, wd)
+        got, fb = _run(g, tipo, esp, g0, target, fech, wd, precisao)
         assert fb < 0.05 * target * np.sum(~wall)   # the MLP is in use almost everywhere
         r = ref.reshape(19, -1)[:, ~wall]
         err = np.abs(got[:, ~wall].astype(np.float64) - r).max(axis=0) / np.abs(r).max()
@@ -211,13 +216,13 @@ def test_closure_step_matches_oracle(orc):
         # rounds to the other bf16 neighbour (the matrix cores sum in another
         # order) changes a cell's nu_t by up to a bf16 step and the flow
         # carries it: a few cells move by ~1e-4, the bulk stays at float32 noise.
-        if target == 1:
+        if target == 1 and precisao == "fp32exact":
             assert err.max() <= 1e-5, err.max()
         else:
             assert err.max() <= 3e-4 and np.percentile(err, 99) <= 1e-5, (
                 err.max(), np.percentile(err, 99))
     # and the closure changed the flow: not the Smagorinsky result
-    smag, _ = _run(g, tipo, esp, g0, 10, None, None)
+    smag, _ = _run(g, tipo, esp, g0, 10, None, None, precisao)
     assert np.abs(smag[:, ~wall] - got[:, ~wall]).max() > 1e-6
 
 
@@ -324,9 +329,9 @@ def test_c5_closure_at_size(orc):
     240x120x96 vehicle tunnel (seed 7 at 0.75 scale, no-slip floor, inlet
     boundary layer, sponge; omega 1.90, Cs 0.1).  From a flow developed by
     1500 plain steps, two closure steps on the GPU agree with the oracle's
-    closure step (float32: 1e-5 after one step; after two, the 99th
-    percentile, with the bf16 rounding flips of the matrix cores bounded
-    at 3e-4), with the wall distance the device computed; the MLP is in use
+    closure step (float32: the 99th percentile within 1e-5 after each step,
+    a cell whose feature or hidden unit rounds to the other bf16 neighbour
+    bounded at 3e-4), with the wall distance the device computed; the MLP is in use
     in at least 90 % of the fluid cell-steps.  Then the float16-storage form
     with stochastic rounding that the workload runs: 600 closure steps stay
     finite with the same fallback bound."""
@@ -370,11 +375,10 @@ def test_c5_closure_at_size(orc):
         err = np.abs(got[k][:, ~wall].astype(np.float64) - r).max(axis=0) / np.abs(r).max()
         print(f"closure step {k + 1}: max {err.max():.2e}, p99 {np.percentile(err, 99):.2e}",
               flush=True)
-        if k == 0:
-            assert err.max() <= 1e-5, err.max()
-        else:
-            assert err.max() <= 3e-4 and np.percentile(err, 99) <= 1e-5, (
-                err.max(), np.percentile(err, 99))
+        # the default 1-ulp arithmetic: a feature that rounds to the other bf16
+        # neighbour moves a cell by up to ~1e-4 already in the first step
+        assert err.max() <= 3e-4 and np.percentile(err, 99) <= 1e-5, (
+            err.max(), np.percentile(err, 99))
 
     s = SolverMI355X.criar(g, precisao="fp16sr")
     try:
