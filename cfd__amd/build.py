@@ -33,10 +33,11 @@ HIP_FLAGS = [
 ]
 # Per-file extras: the voxelizer must not contract a*b+c into fma, since it
 # reproduces IEEE float64 JavaScript arithmetic bit for bit.
-# The lattice kernels round every product before the following add, as the
-# reference's shader text is written: every instantiation of the step (with or
-# without the macro store, the force accumulation or the batch loop around it)
-# then rounds identically, and identically to a plain IEEE float32 CPU build.
+# The lattice kernels round every product before the following add unless the
+# source fuses it explicitly (fmaf): every instantiation of the step (plain,
+# pull-only, with the force accumulation or the batch loop around it, the
+# closure's fallback) then rounds identically, and the exact-rounding mode
+# identically to a plain IEEE float32 CPU build.
 # The lattice kernels also flush float32 subnormals to zero (the WGSL the
 # reference emits leaves subnormals to the implementation, which may flush
 # them): no population is ever subnormal in the workloads, and the step ran

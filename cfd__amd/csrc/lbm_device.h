@@ -2,7 +2,8 @@
 // learned-closure kernels (closure.hip): storage formats, addressing, the
 // (x, y, z) split with periodic neighbour offsets, and the per-cell pieces of
 // the emitted step (moments, equilibrium, non-equilibrium stress, TRT pair
-// collision), written once so that every kernel rounds identically.
+// collision), written once so that every kernel rounds identically (each in
+// its storage format's kFast form).
 #pragma once
 
 #include <hip/hip_runtime.h>
