@@ -185,21 +185,29 @@ __global__ __launch_bounds__(kBlock) void k_pull_macros(const typename S::T* __r
                                       nullptr);
 }
 
+// Index of the calling lane in its wave.
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
 // Workgroup sum of 3 doubles for any BLOCK (wave64 butterfly, then the
 // waves in fixed order through LDS); the result is valid in thread 0.
+// wave_base = the first thread index of the calling wave (wave-uniform).
 template <int BLOCK>
-__device__ __forceinline__ void block_sum_into(double (&v)[3], double (&out)[3]) {
+__device__ __forceinline__ void block_sum_into(double (&v)[3], double (&out)[3],
+                                               uint32_t wave_base) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
   }
   __shared__ double sh[BLOCK / 64][3];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0)
+  const uint32_t wave = wave_base >> 6;
+  const bool first_lane = lane_id() == 0;
+  if (first_lane)
     for (int k = 0; k < 3; ++k) sh[wave][k] = v[k];
   __syncthreads();
-  if (threadIdx.x == 0)
+  if (wave == 0 && first_lane)
     for (int k = 0; k < 3; ++k) {
       double s = sh[0][k];
       for (int q = 1; q < BLOCK / 64; ++q) s += sh[q][k];
@@ -231,19 +239,43 @@ __device__ __forceinline__ void block_sum_into(double (&v)[3], double (&out)[3])
 //
 // Sampled steps also accumulate the body force (float64 per workgroup, one
 // atomic add per workgroup and component into the tunnel's sample slot).
+constexpr uint32_t kNoTile = 0xffffffffu;  // no counter value pending
+
+// True in every thread when `pred` holds in any thread of the workgroup (one
+// ballot per wave through LDS; ends with a barrier, so every thread has read
+// the flags before any later write of them).
+template <int BLOCK>
+__device__ __forceinline__ bool block_any(bool pred, uint32_t wave_base) {
+  __shared__ uint32_t s_any[BLOCK / 64];
+  const bool w = __ballot(pred) != 0;
+  if (lane_id() == 0) s_any[wave_base >> 6] = w ? 1u : 0u;
+  __syncthreads();
+  uint32_t a = 0;
+#pragma unroll
+  for (int q = 0; q < BLOCK / 64; ++q) a |= s_any[q];
+  __syncthreads();
+  return a != 0;
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 512 ? 6 : 5))) void k_batch_run(const BatchTunnel* __restrict__ tun,
                                                      const BatchLaunch L) {
   namespace cg = cooperative_groups;
   cg::grid_group grid = cg::this_grid();
   __shared__ uint32_t s_tile[2];
-  if (blockIdx.x == 0 && threadIdx.x == 0) L.clocks[0] = wall_clock64();
+  // threadIdx.x as a wave-uniform base plus the lane index: the lane index
+  // is recomputed where it is used, so no register holds the thread index
+  // across the loops (held, it was spilled to scratch and reloaded per tile,
+  // and the reload waited for the previous tile's stores)
+  const uint32_t wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
+  const bool leader = threadIdx.x == 0;
+  if (blockIdx.x == 0 && leader) L.clocks[0] = wall_clock64();
   for (int ph = 0; ph < L.nphases; ++ph) {
     const BatchPhase P = cload(L.phases, ph);
     for (uint32_t r = P.r0; r < P.r1; ++r) {
       uint32_t* ctr = L.counters + r;
       const uint32_t grab = (uint32_t)L.grab;
-      if (threadIdx.x == 0) s_tile[0] = atomicAdd(ctr, grab);
+      if (leader) s_tile[0] = atomicAdd(ctr, grab);
       __syncthreads();
       // the tile index is the same in every lane: made wave-uniform, the
       // tile's table, entry and tunnel descriptor come through scalar loads,
@@ -251,7 +283,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 
       uint32_t t0 = __builtin_amdgcn_readfirstlane(s_tile[0]);
       for (uint32_t it = 1; t0 < P.total_tiles; ++it) {
         uint32_t next = 0;
-        if (threadIdx.x == 0) next = atomicAdd(ctr, grab);  // consumed after the tiles
+        if (leader) next = atomicAdd(ctr, grab);  // consumed after the tiles
         const uint32_t tend = min(t0 + grab, P.total_tiles);
         for (uint32_t t = t0; t < tend; ++t) {
         const BatchEntry E = cload(L.entries, P.first + cload(L.tile_entry, P.table + t));
@@ -266,17 +298,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 
         const bool sample =
             L.sample_every > 0 && ((step + 1) % (uint32_t)L.sample_every) == 0;
         if (sample) {
+          // the next tile index goes to LDS now: held in a register across
+          // the force accumulation, which needs every register, it would be
+          // spilled to scratch (and the reload waits for the tile's stores)
+          if (leader && next != kNoTile) s_tile[it & 1] = next;
+          next = kNoTile;
           double f[3] = {0.0, 0.0, 0.0};
-          for (uint32_t cell = c0 + threadIdx.x; cell < c1; cell += BLOCK)
+          for (uint32_t cell = c0 + wave_base + lane_id(); cell < c1; cell += BLOCK)
             step_cell<StoreF32, false, kBlockLog2, true>(cell, src, dst, tipo, nullptr, D.g,
                                                          D.p, f);
           // most tiles hold no body link: their workgroup skips the
           // reduction and the atomics (adding exact zeros changes no sum)
-          const int links =
-              __syncthreads_or(f[0] != 0.0 || f[1] != 0.0 || f[2] != 0.0);
+          const bool links =
+              block_any<BLOCK>(f[0] != 0.0 || f[1] != 0.0 || f[2] != 0.0, wave_base);
           double part[3] = {0.0, 0.0, 0.0};
-          if (links) block_sum_into<BLOCK>(f, part);  // uniform: the whole workgroup
-          if (links && threadIdx.x == 0) {
+          if (links) block_sum_into<BLOCK>(f, part, wave_base);  // uniform: the whole workgroup
+          if (links && leader) {
             const int k = (int)((step + 1) / (uint32_t)L.sample_every) - 1;
             if (k < L.max_samples) {
               double* slot = L.samples + ((size_t)E.tunnel * L.max_samples + k) * 3;
@@ -286,18 +323,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 
             }
           }
         } else {
-          for (uint32_t cell = c0 + threadIdx.x; cell < c1; cell += BLOCK)
+          for (uint32_t cell = c0 + wave_base + lane_id(); cell < c1; cell += BLOCK)
             step_cell<StoreF32, false, kBlockLog2, false>(cell, src, dst, tipo, nullptr,
                                                           D.g, D.p, nullptr);
         }
         }
         // ping-pong slot: a wave still reading the previous slot is not raced
-        if (threadIdx.x == 0) s_tile[it & 1] = next;
+        if (leader && next != kNoTile) s_tile[it & 1] = next;
         __syncthreads();
         t0 = __builtin_amdgcn_readfirstlane(s_tile[it & 1]);
       }
       grid.sync();
-      if (blockIdx.x == 0 && threadIdx.x == 0) L.clocks[1 + r] = wall_clock64();
+      if (blockIdx.x == 0 && leader) L.clocks[1 + r] = wall_clock64();
     }
   }
 }
