@@ -243,10 +243,12 @@ int cfd_batch_run(cfd_batch* b, int steps, int sample_every, const cfd_batch_tun
   BatchEntry* den = nullptr;
   uint32_t* dtab = nullptr;
   uint32_t* dctr = nullptr;
+  uint32_t* dbar = nullptr;
   BCK(hipMalloc(&dph, sizeof(BatchPhase) * phases.size()));
   hipError_t e = hipMalloc(&den, sizeof(BatchEntry) * entries.size());
   if (e == hipSuccess) e = hipMalloc(&dtab, sizeof(uint32_t) * table.size());
   if (e == hipSuccess) e = hipMalloc(&dctr, sizeof(uint32_t) * rounds);
+  if (e == hipSuccess) e = hipMalloc(&dbar, sizeof(uint32_t) * kBarWords);
   if (b->samples) (void)hipFree(b->samples);
   if (b->clocks) (void)hipFree(b->clocks);
   b->samples = nullptr;
@@ -265,12 +267,14 @@ int cfd_batch_run(cfd_batch* b, int steps, int sample_every, const cfd_batch_tun
     e = hipMemcpyAsync(dtab, table.data(), sizeof(uint32_t) * table.size(),
                        hipMemcpyHostToDevice, b->stream);
   if (e == hipSuccess) e = hipMemsetAsync(dctr, 0, sizeof(uint32_t) * rounds, b->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(dbar, 0, sizeof(uint32_t) * kBarWords, b->stream);
   if (e == hipSuccess) e = hipMemsetAsync(b->samples, 0, ns * sizeof(double), b->stream);
   BatchLaunch L{};
   L.phases = dph;
   L.entries = den;
   L.tile_entry = dtab;
   L.counters = dctr;
+  L.bar = dbar;
   L.samples = b->samples;
   L.clocks = b->clocks;
   L.nphases = (int)phases.size();
@@ -288,16 +292,21 @@ int cfd_batch_run(cfd_batch* b, int steps, int sample_every, const cfd_batch_tun
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float ms = 0.f;
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  uint32_t fault = 0;
+  if (e == hipSuccess)
+    e = hipMemcpy(&fault, dbar + kBarFault * kBarStride, sizeof fault, hipMemcpyDeviceToHost);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (dph) (void)hipFree(dph);
   if (den) (void)hipFree(den);
   if (dtab) (void)hipFree(dtab);
   if (dctr) (void)hipFree(dctr);
+  if (dbar) (void)hipFree(dbar);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return bfail(CFD_E_HIP, "cfd_batch_run: %s", hipGetErrorString(e));
   }
+  if (fault) return bfail(CFD_E_HIP, "cfd_batch_run: a grid barrier wait timed out");
   b->tuning = t;
   b->cur ^= steps & 1;
   if (ms_out) *ms_out = ms;

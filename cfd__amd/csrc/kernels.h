@@ -136,12 +136,21 @@ struct BatchLaunch {       // everything k_batch_run reads besides the tunnels
   const BatchEntry* entries;
   const uint32_t* tile_entry;  // per phase: tile -> index of its entry in the phase
   uint32_t* counters;          // one work counter per round, zero at launch
+  uint32_t* bar;               // grid barrier words (kBarWords), zero at launch
   double* samples;             // [tunnel][sample][3]
   unsigned long long* clocks;  // [0] launch start, [1 + r] end of round r
   int nphases, tile_cells, sample_every, max_samples;
   int grab;                    // consecutive tiles taken per counter increment
   uint32_t parity0;            // every tunnel's current state is in pop[parity0]
 };
+// Grid barrier of the batch launch: one word per slot, each on its own
+// 64-byte line (kBarStride uint32 apart).  Slots: per-XCD arrival counters,
+// per-XCD release generations, the global XCD count, per-XCD workgroup
+// counts, and a fault flag (a barrier wait that timed out).
+constexpr uint32_t kBarStride = 16;
+constexpr uint32_t kBarArrive = 0, kBarRelease = kXcds, kBarGlobal = 2 * kXcds,
+                   kBarCount = 2 * kXcds + 1, kBarFault = 3 * kXcds + 1;
+constexpr uint32_t kBarWords = (3 * kXcds + 2) * kBarStride;
 hipError_t batch_max_grid(int block, int* max_blocks_per_cu, int* cus);
 hipError_t launch_batch_run(const BatchTunnel* tun, const BatchLaunch& L, int block, int grid,
                             hipStream_t st);

@@ -241,6 +241,103 @@ __device__ __forceinline__ void block_sum_into(double (&v)[3], double (&out)[3],
 // atomic add per workgroup and component into the tunnel's sample slot).
 constexpr uint32_t kNoTile = 0xffffffffu;  // no counter value pending
 
+// ---- grid barrier of the batch launch
+//
+// cooperative_groups' grid sync makes every workgroup write back its XCD's
+// whole L2 and then invalidate it (an agent-scope release and acquire each):
+// 96 write-backs per XCD per round at 3 workgroups per CU, ~95 us per round
+// measured.  Here the workgroups of an XCD first meet on a per-XCD counter;
+// the last to arrive writes back that XCD's L2 once (every workgroup of the
+// XCD has waited for its own stores to reach that L2 before arriving), meets
+// the other XCDs' last arrivals on a global counter, invalidates, and
+// releases its XCD; every other workgroup then invalidates (agent-scope
+// acquire) for its own CU.  ~24 us per round at 3 workgroups per CU, and
+// bit-identical to the single-tunnel solver.  XCD membership is read from the hardware
+// (HW_REG_XCC_ID) and counted once at launch; if the counts do not cover the
+// grid, every barrier of the launch falls back to the cooperative grid sync.
+// Every wait is bounded: a wait that times out sets the fault word (checked
+// by the host) and lets the launch finish.
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ uint32_t* bar_word(uint32_t* bar, uint32_t slot) {
+  return bar + slot * kBarStride;
+}
+
+// Waits until *w >= target; false after kSpinLimit polls or as soon as the
+// fault word is set (one timed-out wait then ends every later wait at once,
+// so a broken launch finishes instead of timing out round after round).
+__device__ __forceinline__ bool spin_until(uint32_t* bar, uint32_t* w, uint32_t target) {
+  for (uint32_t k = 0; k < kSpinLimit; ++k) {
+    if (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+    if (__hip_atomic_load(bar_word(bar, kBarFault), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return false;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return false;
+}
+
+struct XcdGroup {
+  uint32_t x;     // this workgroup's XCD
+  uint32_t n_x;   // workgroups of the launch on XCD x
+  uint32_t nxcd;  // XCDs holding workgroups of the launch
+  bool ok;        // counts cover the grid: the per-XCD barrier is usable
+};
+
+// Once per launch, by every workgroup (ends with a cooperative grid sync).
+__device__ XcdGroup xcd_group(uint32_t* bar, bool leader, cooperative_groups::grid_group& grid) {
+  XcdGroup g{};
+  g.x = __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15u;  // HW_REG_XCC_ID[3:0]
+  if (leader && g.x < kXcds)
+    __hip_atomic_fetch_add(bar_word(bar, kBarCount + g.x), 1u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  grid.sync();
+  uint32_t total = 0;
+  for (uint32_t k = 0; k < kXcds; ++k) {
+    const uint32_t c =
+        __hip_atomic_load(bar_word(bar, kBarCount + k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    total += c;
+    g.nxcd += c > 0 ? 1u : 0u;
+    if (k == g.x) g.n_x = c;
+  }
+  g.ok = g.x < kXcds && total == gridDim.x;
+  return g;
+}
+
+// Barrier number `gen` (1, 2, ...) of the launch; every workgroup calls it
+// the same number of times.
+__device__ __forceinline__ void batch_barrier(uint32_t* bar, const XcdGroup& g, uint32_t gen,
+                                              bool leader,
+                                              cooperative_groups::grid_group& grid) {
+  if (!g.ok) {
+    grid.sync();
+    return;
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have reached the L2
+  __syncthreads();
+  if (leader) {
+    bool ok;
+    const uint32_t a = __hip_atomic_fetch_add(bar_word(bar, kBarArrive + g.x), 1u,
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (a + 1 == gen * g.n_x) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // write back this XCD's L2
+      __hip_atomic_fetch_add(bar_word(bar, kBarGlobal), 1u, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      ok = spin_until(bar, bar_word(bar, kBarGlobal), gen * g.nxcd);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // invalidate this XCD's L2 (and L1)
+      __hip_atomic_store(bar_word(bar, kBarRelease + g.x), gen, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      ok = spin_until(bar, bar_word(bar, kBarRelease + g.x), gen);
+      // invalidate this CU's L1 (and the XCD's L2 again): an L1-only
+      // invalidate measured stale reads
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    if (!ok)
+      __hip_atomic_store(bar_word(bar, kBarFault), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+}
+
 // True in every thread when `pred` holds in any thread of the workgroup (one
 // ballot per wave through LDS; ends with a barrier, so every thread has read
 // the flags before any later write of them).
@@ -270,6 +367,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 
   const uint32_t wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
   const bool leader = threadIdx.x == 0;
   if (blockIdx.x == 0 && leader) L.clocks[0] = wall_clock64();
+  const XcdGroup xg = xcd_group(L.bar, leader, grid);
   for (int ph = 0; ph < L.nphases; ++ph) {
     const BatchPhase P = cload(L.phases, ph);
     for (uint32_t r = P.r0; r < P.r1; ++r) {
@@ -333,7 +431,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 
         __syncthreads();
         t0 = __builtin_amdgcn_readfirstlane(s_tile[it & 1]);
       }
-      grid.sync();
+      batch_barrier(L.bar, xg, r + 1, leader, grid);
       if (blockIdx.x == 0 && leader) L.clocks[1 + r] = wall_clock64();
     }
   }
