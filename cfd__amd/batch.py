@@ -95,17 +95,20 @@ class LoteMI355X:
         """Times `passos` steps per candidate (state restored afterwards) and
         keeps the fastest; returns {'melhor': ..., 'medidas': [...]}."""
         if candidatos is None:
-            # tile shape x workgroup size x admission cap; the cap is given
-            # in cells (mult x 4096 per resident workgroup) so that every
-            # tile size admits the same amount of work per round
+            # tile shape x workgroup size x resident workgroups per CU (the
+            # most that fit, and one fewer: fewer tiles in flight thrash the
+            # L2 less) x admission cap; the cap is given in cells (mult x 4096
+            # per workgroup that fits) so that every tile size and residency
+            # admits the same amount of work per round
             candidatos = []
             for block in (256, 512):
-                per, cus = self.limite_residencia(block)
-                for tile in (512, 1024, 2048, 4096):
-                    for mult in (8, 32):
-                        candidatos.append({"block": block, "tile_cells": tile, "wg_per_cu": per,
-                                           "tiles_per_grab": 1,
-                                           "max_active_tiles": per * cus * mult * 4096 // tile})
+                pmax, cus = self.limite_residencia(block)
+                for per in sorted({max(1, pmax - 1), pmax}):
+                    for tile in (512, 1024, 2048, 4096):
+                        for mult in (8, 32):
+                            candidatos.append({"block": block, "tile_cells": tile,
+                                               "wg_per_cu": per, "tiles_per_grab": 1,
+                                               "max_active_tiles": pmax * cus * mult * 4096 // tile})
         medidas = []
         for c in candidatos:
             ms = self.rodar(passos, 0, c)

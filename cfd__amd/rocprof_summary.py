@@ -17,6 +17,14 @@ usage: python -m cfd__amd.rocprof_summary --trace D --fetch D --write D
 --last N takes the roofline's average duration from the last N dispatches of
 the main instantiation in the per-dispatch trace (a bench's timed region after
 its warm-up) instead of the average over all of them.
+
+--dram D adds a fourth run,
+  rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B TCC_EA0_WRREQ_WRITE_DRAM_32B -d <dram> ...
+whose counters count DRAM requests in 32-byte units whatever the request size
+(a 128-byte request counts 4), so they give bytes without calibration.  On
+gfx950, FETCH_SIZE's 128-byte term (TCC_BUBBLE) reads zero and every
+128-byte read counts as 64 bytes, which is why the copy-kernel factor is 2;
+the exact counters confirm the calibrated figures.
 """
 from __future__ import annotations
 
@@ -72,6 +80,8 @@ def main():
     ap.add_argument("--out", type=Path, required=True)
     ap.add_argument("--last", type=int, default=0,
                     help="average only the last N dispatches of the main instantiation")
+    ap.add_argument("--dram", type=Path, default=None,
+                    help="directory of a TCC_EA0_RDREQ_DRAM_32B / TCC_EA0_WRREQ_WRITE_DRAM_32B run")
     ap.add_argument("--match", default="",
                     help="the main instantiation's name must also contain this text "
                          "(e.g. 'StoreF32, false, 512')")
@@ -114,12 +124,15 @@ def main():
         f"rocprofv3 --kernel-trace --stats -d <trace> -o run --output-format csv -- {a.command}",
         f"rocprofv3 --pmc FETCH_SIZE -d <fetch> -o run --output-format csv -- {a.command}",
         f"rocprofv3 --pmc WRITE_SIZE -d <write> -o run --output-format csv -- {a.command}",
+        *([f"rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B TCC_EA0_WRREQ_WRITE_DRAM_32B -d <dram> "
+           f"-o run --output-format csv -- {a.command}"] if a.dram is not None else []),
         f"python -m cfd__amd.rocprof_summary --trace <trace> --fetch <fetch> --write <write> "
         f"--kernel {a.kernel} --cells {a.cells} --steps {a.steps} "
         f"--bytes-per-cell {a.bytes_per_cell} "
         f"--copy-bytes {a.copy_bytes} --command '...' --out {a.out}"
         + (f" --last {a.last}" if a.last > 0 else "")
-        + (f" --match '{a.match}'" if a.match else ""),
+        + (f" --match '{a.match}'" if a.match else "")
+        + (" --dram <dram>" if a.dram is not None else ""),
         "```",
         "",
         "## Kernel trace (--stats)",
@@ -141,6 +154,26 @@ def main():
         f"- {a.kernel}: calibrated read {fk * ff * 1024 / 1e6:.1f} MB + write "
         f"{wk * fw * 1024 / 1e6:.1f} MB per launch",
         "",
+    ]
+    if a.dram is not None:
+        dr, dw = _counter(a.dram, "TCC_EA0_RDREQ_DRAM_32B"), _counter(a.dram,
+                                                                  "TCC_EA0_WRREQ_WRITE_DRAM_32B")
+        _, crd = _pick(dr, "k_copy")
+        _, cwr = _pick(dw, "k_copy")
+        rk, wk2 = same(dr), same(dw)
+        lines += [
+            f"- exact DRAM requests (TCC_EA0_RDREQ_DRAM_32B / TCC_EA0_WRREQ_WRITE_DRAM_32B, "
+            f"32-byte units, no calibration): k_copy read {crd * 32 / a.copy_bytes:.4f} and "
+            f"write {cwr * 32 / a.copy_bytes:.4f} of its bytes; {a.kernel} read "
+            f"{rk * 32 / 1e6:.1f} MB + write {wk2 * 32 / 1e6:.1f} MB per launch = "
+            f"{(rk + wk2) * 32 / (a.bytes_per_cell * a.cells * a.steps):.4f} of algorithmic "
+            f"(read {rk * 32 / (a.bytes_per_cell / 2 * a.cells * a.steps):.4f}, write "
+            f"{wk2 * 32 / (a.bytes_per_cell / 2 * a.cells * a.steps):.4f})",
+            "",
+        ]
+    else:
+        lines += [""]
+    lines += [
         "## Roofline",
         "",
         f"- algorithmic_bytes_per_launch: {alg}  ({a.bytes_per_cell} B/cell x {a.cells} cells"

@@ -284,7 +284,9 @@ int cfd_batch_create(const cfd_tunnel_spec *specs, int n, cfd_batch **out);
 /* Back to the free-stream equilibrium (e.g. after autotuning runs). */
 int cfd_batch_reset(cfd_batch *b);
 /* One cooperative launch running `steps` steps of every tunnel; tuning may
- * be NULL (last used / default).  ms_out: device time of the launch. */
+ * be NULL (last used / default).  ms_out: device time of the launch.
+ * CFD_E_HIP also when a wait of the launch's grid barrier timed out (the
+ * launch then finishes early and the tunnels' state is not to be used). */
 int cfd_batch_run(cfd_batch *b, int steps, int sample_every, const cfd_batch_tuning *t,
                   double *ms_out);
 /* forces: n * n_samples * 3 doubles (nullable); finish_ms: n doubles, time
