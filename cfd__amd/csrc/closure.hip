@@ -4,12 +4,18 @@
 // Inference runs on the matrix cores: bf16 operands, float32 accumulation,
 // v_mfma_f32_16x16x32_bf16.  A wave steps 64 cells; each lane holds its own
 // cell's 8 features as the B operand (k = 8 * (lane / 16) + v), and layer 1 is
-// evaluated as four 16-cell blocks by masking the A operand (W1) to one lane
-// group at a time, so no lane ever gathers another lane's features.  A 16x16
-// result tile leaves hidden unit 4 * (lane / 16) + v of cell (lane % 16) in
-// each lane, which is already the B layout of the next layer once the next
-// weight matrix has its input index permuted (done on the host).  The single
-// output lands in lanes 0-15 and one shuffle returns it to the owning lane.
+// evaluated as four 16-cell blocks with the A operand (W1) read from a zero
+// row in every lane group but one, so no lane ever gathers another lane's
+// features.  A 16x16 result tile leaves hidden unit 4 * (lane / 16) + v of
+// cell (lane % 16) in each lane, which is already the B layout of the next
+// layer once the next weight matrix has its input index permuted (done on
+// the host).  Layer 3 puts w3 in every row of A, so every lane group receives
+// each cell's output and the owning lane keeps its own: no shuffle.
+//
+// The network runs in base 2 (pack_closure): layer 1 and 2 produce
+// u = -log2(e) z, whose SiLU is u / (1 + 2^u) = -log2(e) SiLU(z), and w3
+// carries the -ln 2 back, so a SiLU is one exponential, one add, one
+// reciprocal and one multiply.
 //
 // The closure step (k_step_closure) is the step of lbm_kernels.hip with the
 // eddy viscosity taken from the MLP: features are the non-equilibrium stress
@@ -44,8 +50,10 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------ inference
 
-__device__ __forceinline__ float silu_fast(float a) {
-  return a * __builtin_amdgcn_rcpf(1.0f + __expf(-a));
+// SiLU in base 2 on u = -log2(e) z: u / (1 + 2^u) = -log2(e) SiLU(z).  For
+// u beyond the exponential's range 2^u is +inf and the result 0.
+__device__ __forceinline__ float silu2_fast(float u) {
+  return u * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(u));
 }
 
 __device__ __forceinline__ float softplus_fast(float z) {
@@ -64,8 +72,8 @@ __device__ __forceinline__ v8bf pack_silu(const v4f& a, const v4f& b) {
   v8bf r;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    r[v] = (__bf16)silu_fast(a[v]);
-    r[v + 4] = (__bf16)silu_fast(b[v]);
+    r[v] = (__bf16)silu2_fast(a[v]);
+    r[v + 4] = (__bf16)silu2_fast(b[v]);
   }
   return r;
 }
@@ -74,33 +82,29 @@ __device__ __forceinline__ v8bf pack_silu(const v4f& a, const v4f& b) {
 // bf16 (zero for lanes that do not use the result).  All 64 lanes must call.
 __device__ __forceinline__ float mlp_wave(const v8bf& xb, const MlpPacked& W) {
   const int l = threadIdx.x & 63, grp = l >> 4, row = l & 15;
-  const v8bf zero = {};
   const v4f zero4 = {0.f, 0.f, 0.f, 0.f};
+  // operands shared by the four 16-cell blocks
+  const v8bf w2a = ld8(&W.w2[0][row][8 * grp]), w2b = ld8(&W.w2[1][row][8 * grp]);
+  const v8bf a3 = ld8(&W.w3[8 * grp]);
+  const v4f b1a = ld4(&W.b1[4 * grp]), b1b = ld4(&W.b1[16 + 4 * grp]);
+  const v4f b2a = ld4(&W.b2[4 * grp]), b2b = ld4(&W.b2[16 + 4 * grp]);
   float mine = 0.0f;
-  // not unrolled: the weights are re-read from LDS per 16-cell block instead
-  // of being held in registers across all four (one more wave per SIMD)
 #pragma unroll 1
   for (int gs = 0; gs < 4; ++gs) {
-    const bool sel = grp == gs;
-    // layer 1, cells 16 gs .. 16 gs + 15: A = W1 rows in lane group gs only
-    const v8bf a0 = sel ? ld8(W.w1[0][row]) : zero;
-    const v8bf a1 = sel ? ld8(W.w1[1][row]) : zero;
-    const v4f h0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, xb, ld4(&W.b1[4 * grp]), 0, 0, 0);
-    const v4f h1 =
-        __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, xb, ld4(&W.b1[16 + 4 * grp]), 0, 0, 0);
+    // layer 1, cells 16 gs .. 16 gs + 15: W1 rows in lane group gs, the zero
+    // row 16 in the others
+    const int r1 = grp == gs ? row : 16;
+    const v4f h0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(W.w1[0][r1]), xb, b1a, 0, 0, 0);
+    const v4f h1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(W.w1[1][r1]), xb, b1b, 0, 0, 0);
     const v8bf x2 = pack_silu(h0, h1);
     // layer 2
-    const v4f g0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(&W.w2[0][row][8 * grp]), x2,
-                                                           ld4(&W.b2[4 * grp]), 0, 0, 0);
-    const v4f g1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(&W.w2[1][row][8 * grp]), x2,
-                                                           ld4(&W.b2[16 + 4 * grp]), 0, 0, 0);
+    const v4f g0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2a, x2, b2a, 0, 0, 0);
+    const v4f g1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2b, x2, b2b, 0, 0, 0);
     const v8bf x3 = pack_silu(g0, g1);
-    // layer 3: row 0 of A is w3, the other rows zero
-    const v8bf a3 = row == 0 ? ld8(&W.w3[8 * grp]) : zero;
+    // layer 3: every row of A is w3, so every row of the result holds the
+    // outputs of the 16 cells; the lanes of group gs own them
     const v4f zz = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, x3, zero4, 0, 0, 0);
-    // the result of cell 16 gs + j sits in lane j
-    const float z = __shfl(zz[0], l - 16 * gs, 64);
-    if (sel) mine = z;
+    if (grp == gs) mine = zz[0];
   }
   return mine + W.b3;
 }
@@ -149,12 +153,16 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   const bool wall = is_wall(ct);
   if (!wall && (tb & NEAR_WALL)) bounce_back<false>(gi, sb, own, tipo, in, p, ct, nullptr);
 
+  // the arithmetic runs in the stored units (g * unit), as the plain step's
   constexpr bool F = S::kFast;
-  const Moments M = moments<F>(gi);
+  constexpr float unit = unit_of<S>();
+  const Moments M = moments<F, S::kScaleLog2>(gi);
   float pab[6];
-  {
-    // the equilibrium is recomputed after the MLP rather than held across it
-    // (19 registers less at the matrix instructions: one more wave per SIMD)
+  if constexpr (F) {
+    noneq_stress_fast(M, pab);
+  } else {
+    // the exact form's equilibrium is recomputed after the MLP rather than
+    // held across it (fewer registers at the matrix instructions)
     float e[Q];
     equilibrium<F>(M, e);
     noneq_stress(gi, e, pab);
@@ -164,12 +172,13 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   const bool fluid = live && !wall && ct != ESPELHO_Y && ct != ESPELHO_Z && ct != ENTRADA &&
                      ct != SAIDA;
 
-  // features, normalized; outside the trained box the cell keeps tau_s
+  // features, normalized; outside the trained box the cell keeps tau_s.
+  // Pi / rho and Q_s / rho are ratios of two values in the same units.
   float x[kMlpIn];
-  const float inv_rho = 1.0f / M.rho;
+  const float inv_rho = F ? __builtin_amdgcn_rcpf(M.rho) : 1.0f / M.rho;
 #pragma unroll
   for (int k = 0; k < 6; ++k) x[k] = pab[k] * inv_rho;
-  x[6] = 1.5f * qs * inv_rho / tau_s;
+  x[6] = F ? 1.5f * qs * inv_rho * __builtin_amdgcn_rcpf(tau_s) : 1.5f * qs * inv_rho / tau_s;
   x[7] = (float)wall8[cell] * 0.125f;
   bool inside = true;
   v8bf xb;
@@ -202,23 +211,33 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   const float tau = ok ? p.tau0 + 3.0f * nu : tau_s;
 
   float d[Q];
-  if (!boundary_cell(ct, gi, M, c.z, p, d)) {
-    float e[Q];
-    equilibrium<F>(M, e);
-    trt<F>(gi, e, tau, p, d);
-    sponge<F>(c.x, p, d);
+  if (!boundary_cell(ct, gi, M, c.z, p, d, unit)) {
+    if constexpr (F) {
+      collide_fast(gi, M, tau, p, d);
+    } else {
+      float e[Q];
+      equilibrium<F>(M, e);
+      trt<F>(gi, e, tau, p, d);
+    }
+    sponge<F>(c.x, p, d, unit);
   }
   // wall lanes of a wave that also holds other cells store zeros (never
   // read), so that every line of dst is written whole (no partial-line fetch)
   const bool mixed = __ballot(live && !wall) != 0;
-  if (live && (!wall || mixed)) {
-    const uint32_t h = S::kStochastic ? sr_hash(cell, p.sr_seed) : 0u;
+  if (live) {
+    if (!wall) {
+      const uint32_t h = S::kStochastic ? sr_hash(cell, p.sr_seed) : 0u;
+      f16x2 r = {};
+      out.st_first(own, d[0], h, r);
 #pragma unroll
-    for (int i = 0; i < Q; ++i)
-      out.st(i, own, wall ? 0.0f : d[i], h);
+      for (int i = 1; i < Q; i += 2) out.st_pair(i, own, d[i], d[i + 1], h, r);
+    } else if (mixed) {
+      out.st_zero(own);
+    }
   }
   if (MACROS && live)
-    macros[cell] = wall ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(M.ux, M.uy, M.uz, M.delta);
+    macros[cell] = wall ? make_float4(0.f, 0.f, 0.f, 0.f)
+                        : make_float4(M.ux, M.uy, M.uz, M.delta * (1.0f / unit));
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t n = 0;
@@ -588,18 +607,24 @@ int efail(int code, const char* what, hipError_t e) {
 
 namespace cfd {
 
+// The base-2 form of the network (float32 products, then bf16 for the
+// matrix operands): W1, b1, b2 times -log2(e), so layers 1 and 2 produce
+// u = -log2(e) z; W2 unchanged (its input is -log2(e) SiLU and its output is
+// scaled like its bias); w3 times -ln 2.  Row 16 of each W1 tile stays zero.
 void pack_closure(const cfd_closure& c, MlpPacked* out) {
   std::memset(out, 0, sizeof *out);
   const cfd_mlp& m = c.mlp;
+  const float nl2e = -kLog2e, nln2 = -kLn2;
   for (int t = 0; t < 2; ++t)
     for (int i = 0; i < 16; ++i) {
-      for (int v = 0; v < 8; ++v) out->w1[t][i][v] = bf16_rne(m.w1[(16 * t + i) * 8 + v]);
+      for (int v = 0; v < 8; ++v)
+        out->w1[t][i][v] = bf16_rne(nl2e * m.w1[(16 * t + i) * 8 + v]);
       for (int k = 0; k < 32; ++k) out->w2[t][i][k] = bf16_rne(m.w2[(16 * t + i) * 32 + perm_k(k)]);
     }
-  for (int k = 0; k < 32; ++k) out->w3[k] = bf16_rne(m.w3[perm_k(k)]);
+  for (int k = 0; k < 32; ++k) out->w3[k] = bf16_rne(nln2 * m.w3[perm_k(k)]);
   for (int o = 0; o < 32; ++o) {
-    out->b1[o] = m.b1[o];
-    out->b2[o] = m.b2[o];
+    out->b1[o] = nl2e * m.b1[o];
+    out->b2[o] = nl2e * m.b2[o];
   }
   out->b3 = m.b3;
   out->out_scale = c.out_scale;

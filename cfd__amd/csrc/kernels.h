@@ -165,8 +165,13 @@ constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + kMlpHidden * kMlpH
 // k = 8q + v -> (v < 4 ? 4q + v : 16 + 4q + v - 4), which is how a 16x16
 // result tile is spread over the lanes; w2 and w3 are stored with their input
 // index permuted accordingly.
+// log2(e) and ln(2) in float32: the closure network runs in base 2.
+constexpr float kLog2e = 1.44269504f;
+constexpr float kLn2 = 0.693147181f;
+// The closure MLP as the matrix-core kernels read it (bf16 weights of the
+// base-2 form, see pack_closure).
 struct MlpPacked {
-  uint16_t w1[2][16][8];    // M-tile m, row i = hidden 16m + i, input v
+  uint16_t w1[2][17][8];    // M-tile m, row i = hidden 16m + i, input v; row 16 zero
   uint16_t w2[2][16][32];   // M-tile m, row i = output 16m + i, permuted input k
   uint16_t w3[32];          // permuted input k
   float b1[32], b2[32];

@@ -64,6 +64,14 @@ struct StoreF32 {
   static __device__ __forceinline__ void st(P p, float v) { *p = v; }
   static __device__ __forceinline__ float dec(Bits b) { return __uint_as_float(b); }
   static __device__ __forceinline__ Bits enc(float v) { return __float_as_uint(v); }
+  // the step's arithmetic runs in the stored units: value * 2^kScaleLog2
+  static constexpr int kScaleLog2 = 0;
+  template <class P>
+  static __device__ __forceinline__ float ld_s(P p) { return *p; }
+  template <class P>
+  static __device__ __forceinline__ void st_s(P p, float v) { *p = v; }
+  static __device__ __forceinline__ float dec_s(Bits b) { return __uint_as_float(b); }
+  static __device__ __forceinline__ Bits enc_s(float v) { return __float_as_uint(v); }
 };
 
 struct StoreF32X : StoreF32 {
@@ -90,6 +98,21 @@ struct StoreF16 {
   static __device__ __forceinline__ Bits enc(float v) {
     return __builtin_bit_cast(Bits, static_cast<T>(v * 4096.0f));
   }
+  // The step keeps the populations in the stored units (g * 2^12) through
+  // its arithmetic: scaling by a power of two commutes with every rounding,
+  // so the collision rounds exactly as it would on g itself, without a
+  // multiply per load and per store.
+  static constexpr int kScaleLog2 = 12;
+  template <class P>
+  static __device__ __forceinline__ float ld_s(P p) { return static_cast<float>(*p); }
+  template <class P>
+  static __device__ __forceinline__ void st_s(P p, float v) { *p = static_cast<T>(v); }
+  static __device__ __forceinline__ float dec_s(Bits b) {
+    return static_cast<float>(__builtin_bit_cast(T, b));
+  }
+  static __device__ __forceinline__ Bits enc_s(float v) {
+    return __builtin_bit_cast(Bits, static_cast<T>(v));
+  }
 };
 
 // The same float16 storage with stochastic rounding in the step's stores: the
@@ -106,22 +129,12 @@ typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
 struct StoreF16S : StoreF16 {
   static constexpr bool kStochastic = true;
-  static __device__ __forceinline__ T round_sr(float v, uint32_t rnd) {
-    const f16x2 zero = {};
-    return __builtin_amdgcn_cvt_sr_f16_f32(zero, v * 4096.0f, rnd, false)[0];
-  }
-  template <class P>
-  static __device__ __forceinline__ void st_sr(P p, float v, uint32_t rnd) {
-    *p = round_sr(v, rnd);
-  }
-  static __device__ __forceinline__ Bits enc_sr(float v, uint32_t rnd) {
-    return __builtin_bit_cast(Bits, round_sr(v, rnd));
-  }
 };
 
 // A 32-bit random word for population i of a cell: one avalanche hash per
-// cell and step, then a 24-bit multiply by an odd per-direction constant
-// (full rate, unlike a 32-bit multiply), folded with the hash.
+// cell and step, then the Weyl sequence h + i * 0x9E3779B9 (2^32 / golden
+// ratio): every word is as uniform as the hash, and the 19 words of a cell
+// are spread evenly around the circle, for one add per population.
 __device__ __forceinline__ uint32_t sr_hash(uint32_t cell, uint32_t seed) {
   uint32_t h = cell * 0x9E3779B1u ^ seed;
   h ^= h >> 16;
@@ -132,16 +145,13 @@ __device__ __forceinline__ uint32_t sr_hash(uint32_t cell, uint32_t seed) {
   return h;
 }
 __device__ __forceinline__ uint32_t sr_bits(uint32_t h, int i) {
-  const uint32_t k = ((0x9E3779u * (2u * (uint32_t)i + 1u)) & 0xFFFFFFu) | 1u;
-  return __umul24(h, k) ^ h;
+  return h + (uint32_t)i * 0x9E3779B9u;
 }
 
-// Store population i of cell c: rounded to nearest, or stochastically for
-// StoreF16S (h = sr_hash of the cell for this step).
-template <class S, class P>
-__device__ __forceinline__ void store_pop(P p, float v, uint32_t h, int i) {
-  if constexpr (S::kStochastic) S::st_sr(p, v, sr_bits(h, i));
-  else S::st(p, v);
+// 2^kScaleLog2 of a storage format: the step's unit of a population.
+template <class S>
+__device__ __forceinline__ constexpr float unit_of() {
+  return (float)(1u << S::kScaleLog2);
 }
 
 // ------------------------------------------------------------------ indexing
@@ -217,7 +227,8 @@ __device__ __forceinline__ const T* at_byte(const T* base, uint32_t off) {
 // set, the direction's base as the scalar offset and the cell's byte offset
 // as the only per-lane address (no 64-bit address arithmetic per access).
 // SoA sets (which may exceed 4 GiB) use a wave-uniform 64-bit base per
-// direction and the same 32-bit byte offset.
+// direction and the same 32-bit byte offset.  Values are in the stored units
+// (g * unit_of<S>()), the units of the step's arithmetic.
 constexpr int kBufferWord3 = 0x00020000;  // raw dword access, gfx9-family resource
 
 template <class S, int LB>
@@ -238,29 +249,62 @@ struct PopView {
   }
   __device__ __forceinline__ float ld(int i, uint32_t off) const {
     if constexpr (LB == 0) {
-      return S::ld(at_byte(base + (size_t)i * slice, off));
+      return S::ld_s(at_byte(base + (size_t)i * slice, off));
     } else if constexpr (sizeof(T) == 4) {
-      return S::dec((typename S::Bits)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)off,
-                                                                           dir_off(i), 0));
+      return S::dec_s((typename S::Bits)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)off,
+                                                                             dir_off(i), 0));
     } else {
-      return S::dec((typename S::Bits)__builtin_amdgcn_raw_buffer_load_b16(rsrc, (int)off,
-                                                                           dir_off(i), 0));
+      return S::dec_s((typename S::Bits)__builtin_amdgcn_raw_buffer_load_b16(rsrc, (int)off,
+                                                                             dir_off(i), 0));
     }
   }
-  // population i of the cell at byte offset off: rounded to nearest, or
-  // stochastically for StoreF16S (h = sr_hash of the cell for this step)
-  __device__ __forceinline__ void st(int i, uint32_t off, float v, uint32_t h) const {
+  // raw bits of population i of the cell at byte offset off
+  __device__ __forceinline__ void st_bits(int i, uint32_t off, typename S::Bits b) const {
     if constexpr (LB == 0) {
-      store_pop<S>(at_byte(base + (size_t)i * slice, off), v, h, i);
+      *(GLOBAL typename S::Bits*)at_byte(base + (size_t)i * slice, off) = b;
+    } else if constexpr (sizeof(T) == 4) {
+      __builtin_amdgcn_raw_buffer_store_b32((int)b, rsrc, (int)off, dir_off(i), 0);
     } else {
-      typename S::Bits b;
-      if constexpr (S::kStochastic) b = S::enc_sr(v, sr_bits(h, i));
-      else b = S::enc(v);
-      if constexpr (sizeof(T) == 4)
-        __builtin_amdgcn_raw_buffer_store_b32((int)b, rsrc, (int)off, dir_off(i), 0);
-      else
-        __builtin_amdgcn_raw_buffer_store_b16((short)b, rsrc, (int)off, dir_off(i), 0);
+      __builtin_amdgcn_raw_buffer_store_b16((short)b, rsrc, (int)off, dir_off(i), 0);
     }
+  }
+  // Stores of the cell at byte offset off, values in step units: rounded to
+  // nearest, or stochastically for StoreF16S (h = sr_hash of the cell for
+  // this step).  st_first stores population 0, st_pair the opposite pair
+  // (i, i + 1): float16 sets convert the pair into the two halves of one
+  // register r (to nearest with one packed conversion, stochastically with one
+  // conversion into each half, so no register is cleared per value).
+  __device__ __forceinline__ void st_first(uint32_t off, float v, uint32_t h, f16x2& r) const {
+    if constexpr (sizeof(T) == 4) {
+      st_bits(0, off, S::enc_s(v));
+    } else if constexpr (S::kStochastic) {
+      r = __builtin_amdgcn_cvt_sr_f16_f32(r, v, sr_bits(h, 0), false);
+      st_bits(0, off, __builtin_bit_cast(uint32_t, r) & 0xffffu);
+    } else {
+      st_bits(0, off, S::enc_s(v));
+    }
+  }
+  __device__ __forceinline__ void st_pair(int i, uint32_t off, float a, float b, uint32_t h,
+                                          f16x2& r) const {
+    if constexpr (sizeof(T) == 4) {
+      st_bits(i, off, S::enc_s(a));
+      st_bits(i + 1, off, S::enc_s(b));
+    } else {
+      if constexpr (S::kStochastic) {
+        r = __builtin_amdgcn_cvt_sr_f16_f32(r, a, sr_bits(h, i), false);
+        r = __builtin_amdgcn_cvt_sr_f16_f32(r, b, sr_bits(h, i + 1), true);
+      } else {
+        r = f16x2{(_Float16)a, (_Float16)b};
+      }
+      const uint32_t w = __builtin_bit_cast(uint32_t, r);
+      st_bits(i, off, w & 0xffffu);
+      st_bits(i + 1, off, w >> 16);
+    }
+  }
+  // zeros in every population of the cell (wall lanes of a mixed wave)
+  __device__ __forceinline__ void st_zero(uint32_t off) const {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) st_bits(i, off, 0);
   }
 };
 
@@ -364,7 +408,8 @@ __device__ __forceinline__ float dotc(int i, float vx, float vy, float vz) {
 //
 // FORCE also accumulates, into f, the momentum exchange c_j 2 (g_j + w_j) of
 // every SOLIDO link of a FLUIDO cell (wgsl.js:358-379), j = opp(i), from the
-// re-loaded g_j before the wall term is added.
+// re-loaded g_j before the wall term is added.  gi is in the step's units
+// (g * unit_of<S>()).
 template <bool FORCE, class S, int LB>
 __device__ __forceinline__ void bounce_back(float (&gi)[Q], const uint32_t (&sb)[Q],
                                             uint32_t own, const GLOBAL uint8_t* __restrict__ tipo,
@@ -393,62 +438,119 @@ __device__ __forceinline__ void bounce_back(float (&gi)[Q], const uint32_t (&sb)
                           moving ? p.belt[2] : 0.f);
     if (FORCE && ((solid_m >> i) & 1u) && ct == FLUIDO) {
       const int j = opp(i);
-      const double q = 2.0 * ((double)gi[i] + wd(j));
+      const double q = 2.0 * ((double)gi[i] * (1.0 / (double)unit_of<S>()) + wd(j));
       if (cx(j)) f[0] += cx(j) > 0 ? q : -q;
       if (cy(j)) f[1] += cy(j) > 0 ? q : -q;
       if (cz(j)) f[2] += cz(j) > 0 ? q : -q;
     }
-    gi[i] = gi[i] + kwall(i) * cu;
+    gi[i] = gi[i] + (unit_of<S>() * kwall(i)) * cu;
   }
 }
 
 // ------------------------------------------------------------------ cell physics
 
-// Moments of the pulled populations (ir.js:192-213), left-to-right sums as
-// emitted: delta = sum g_i, rho = 1 + delta, m = sum c_i g_i.
+// Moments of the pulled populations (ir.js:192-213).
+//
+// Exact form: left-to-right sums as emitted, delta = sum g_i, rho = 1 + delta,
+// m = sum c_i g_i, u = m / rho.
+//
+// FAST form, in the step's units (g * 2^SL2, so rho = 2^SL2 + delta): delta
+// as emitted; the opposite pairs (2k-1, 2k) give sums a_k and differences
+// b_k, then m_a = sum_k c_a(2k-1) b_k and the second moments S_ab =
+// sum_i c_ia c_ib g_i = sum_k c_a(2k-1) c_b(2k-1) a_k: 63 adds give delta,
+// the momentum and all six second moments (the emitted form takes 45 for
+// delta and m alone, and Pi another 54 through the 19 equilibria).
 struct Moments {
   float delta, rho, ux, uy, uz, uu;
+  float m[3];   // FAST: momentum rho u
+  float s2[6];  // FAST: sum c_ia c_ib g_i in the order xx, yy, zz, xy, xz, yz
 };
 
-template <bool FAST = false>
+CFD_HD constexpr int c_axis(int i, int a) { return a == 0 ? cx(i) : (a == 1 ? cy(i) : cz(i)); }
+
+// Signed sum of v[k] over the pairs k = 1..9 with weight c(2k-1) in {-1, 0, 1},
+// left to right.
+template <class C>
+__device__ __forceinline__ float pair_sum(const float (&v)[10], C coef) {
+  float acc = 0.0f;
+  bool first = true;
+#pragma unroll
+  for (int k = 1; k <= 9; ++k) {
+    const int c = coef(2 * k - 1);
+    if (c == 0) continue;
+    if (first) {
+      acc = c > 0 ? v[k] : -v[k];
+      first = false;
+    } else {
+      acc = c > 0 ? acc + v[k] : acc - v[k];
+    }
+  }
+  return acc;
+}
+
+template <bool FAST = false, int SL2 = 0>
 __device__ __forceinline__ Moments moments(const float (&gi)[Q]) {
   Moments M;
-  float delta = gi[0];
-#pragma unroll
-  for (int i = 1; i < Q; ++i) delta = delta + gi[i];
-  M.delta = delta;
-  M.rho = 1.0f + delta;
-  float m[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    float acc = 0.0f;
-    bool first = true;
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      const int ca = a == 0 ? cx(i) : (a == 1 ? cy(i) : cz(i));
-      if (ca == 0) continue;
-      if (first) {
-        acc = ca > 0 ? gi[i] : -gi[i];
-        first = false;
-      } else {
-        acc = ca > 0 ? acc + gi[i] : acc - gi[i];
-      }
-    }
-    m[a] = acc;
-  }
   if constexpr (FAST) {
+    float a[10], b[10];
+#pragma unroll
+    for (int k = 1; k <= 9; ++k) {
+      a[k] = gi[2 * k - 1] + gi[2 * k];
+      b[k] = gi[2 * k - 1] - gi[2 * k];
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      M.m[ax] = pair_sum(b, [ax](int i) { return c_axis(i, ax); });
+      M.s2[ax] = pair_sum(a, [ax](int i) { return c_axis(i, ax) * c_axis(i, ax); });
+    }
+    M.s2[3] = pair_sum(a, [](int i) { return cx(i) * cy(i); });
+    M.s2[4] = pair_sum(a, [](int i) { return cx(i) * cz(i); });
+    M.s2[5] = pair_sum(a, [](int i) { return cy(i) * cz(i); });
+    // delta in the emitted order: rho = 1 + delta is the divisor of u, and a
+    // cell whose rho is small (a far-from-equilibrium state) would magnify
+    // any other rounding of it
+    float delta = gi[0];
+#pragma unroll
+    for (int i = 1; i < Q; ++i) delta = delta + gi[i];
+    M.delta = delta;
+    M.rho = (float)(1u << SL2) + delta;
     const float r = __builtin_amdgcn_rcpf(M.rho);
-    M.ux = m[0] * r;
-    M.uy = m[1] * r;
-    M.uz = m[2] * r;
+    M.ux = M.m[0] * r;
+    M.uy = M.m[1] * r;
+    M.uz = M.m[2] * r;
     M.uu = fmaf(M.ux, M.ux, fmaf(M.uy, M.uy, M.uz * M.uz));
+    return M;
   } else {
+    static_assert(SL2 == 0, "the exact form runs on float32 storage");
+    float delta = gi[0];
+#pragma unroll
+    for (int i = 1; i < Q; ++i) delta = delta + gi[i];
+    M.delta = delta;
+    M.rho = 1.0f + delta;
+    float m[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      float acc = 0.0f;
+      bool first = true;
+#pragma unroll
+      for (int i = 1; i < Q; ++i) {
+        const int ca = c_axis(i, ax);
+        if (ca == 0) continue;
+        if (first) {
+          acc = ca > 0 ? gi[i] : -gi[i];
+          first = false;
+        } else {
+          acc = ca > 0 ? acc + gi[i] : acc - gi[i];
+        }
+      }
+      m[ax] = acc;
+    }
     M.ux = m[0] / M.rho;
     M.uy = m[1] / M.rho;
     M.uz = m[2] / M.rho;
     M.uu = M.ux * M.ux + M.uy * M.uy + M.uz * M.uz;
+    return M;
   }
-  return M;
 }
 
 // Shifted equilibrium (ir.js:216-232).  FAST: the same polynomial as
@@ -500,6 +602,22 @@ __device__ __forceinline__ void noneq_stress(const float (&gi)[Q], const float (
     }
     pab[k] = acc;
   }
+}
+
+// FAST: Pi from the moments.  The shifted equilibrium's second moment is
+// sum c_ia c_ib e_i = delta/3 delta_ab + rho u_a u_b (the lattice's
+// fourth-order isotropy, sum w c_a c_b c_c c_d = (delta_ab delta_cd +
+// delta_ac delta_bd + delta_ad delta_bc) / 9), so Pi_ab = S_ab - m_a u_b -
+// delta/3 delta_ab without forming the 19 equilibria (the same quantity as
+// noneq_stress up to float32 rounding).
+__device__ __forceinline__ void noneq_stress_fast(const Moments& M, float (&pab)[6]) {
+  const float d3 = M.delta * 0.333333333f;
+  pab[0] = fmaf(-M.m[0], M.ux, M.s2[0] - d3);
+  pab[1] = fmaf(-M.m[1], M.uy, M.s2[1] - d3);
+  pab[2] = fmaf(-M.m[2], M.uz, M.s2[2] - d3);
+  pab[3] = fmaf(-M.m[0], M.uy, M.s2[3]);
+  pab[4] = fmaf(-M.m[0], M.uz, M.s2[4]);
+  pab[5] = fmaf(-M.m[1], M.uz, M.s2[5]);
 }
 
 // Q_s = sqrt(2 Pi:Pi) (ir.js:286-289).
@@ -562,27 +680,67 @@ __device__ __forceinline__ void trt(const float (&gi)[Q], const float (&e)[Q], f
   }
 }
 
+// FAST TRT (ir.js:298-323) from the moments, per opposite pair (i, j = i + 1)
+// with cu = c_i . u (c_j = -c_i): the equilibrium's pair sum
+// e_i + e_j = w (2 delta + rho (9 cu^2 - 3 uu)) and difference
+// e_i - e_j = 6 w rho cu, so no equilibrium array is formed.  Then, as trt<true>,
+// g -/+ (omega+/2) (sum_g - sum_e) -/+ (omega-/2) (dif_g - dif_e).
+__device__ __forceinline__ void collide_fast(const float (&gi)[Q], const Moments& M, float tau,
+                                             const StepParams& p, float (&d)[Q]) {
+  const float omP = __builtin_amdgcn_rcpf(tau);
+  const float omM =
+      __builtin_amdgcn_rcpf(fmaf(p.magic, __builtin_amdgcn_rcpf(tau - 0.5f), 0.5f));
+  const float hp = 0.5f * omP, hm = 0.5f * omM;
+  const float e0 = wf(0) * fmaf(-M.rho, 1.5f * M.uu, M.delta);
+  d[0] = fmaf(-omP, gi[0] - e0, gi[0]);
+  // t = 2 delta - 3 rho uu; per weight class A = w t, B = 9 w rho, C = 6 w rho
+  const float t = fmaf(-3.0f * M.rho, M.uu, 2.0f * M.delta);
+  const float Aa = wf(1) * t, Ad = wf(7) * t;
+  const float Ba = (9.0f * wf(1)) * M.rho, Bd = (9.0f * wf(7)) * M.rho;
+  const float Ca = (6.0f * wf(1)) * M.rho, Cd = (6.0f * wf(7)) * M.rho;
+#pragma unroll
+  for (int i = 1; i < Q; i += 2) {
+    const int j = i + 1;
+    const bool axis = i <= 6;
+    const float cu = dotc(i, M.ux, M.uy, M.uz);
+    const float ep = fmaf(axis ? Ba : Bd, cu * cu, axis ? Aa : Ad);
+    const float em = (axis ? Ca : Cd) * cu;
+    // g_i + (g_j - e+): not the moments' pair sum, which would otherwise stay
+    // live from the moments to here (18 registers, two waves per SIMD)
+    const float sp = gi[i] + (gi[j] - ep);
+    const float sm = gi[i] - (gi[j] + em);
+    d[i] = fmaf(-hm, sm, fmaf(-hp, sp, gi[i]));
+    d[j] = fmaf(hm, sm, fmaf(-hp, sp, gi[j]));
+  }
+}
+
 // Sponge (ir.js:458-478): mix(dst, teq, sigma^2).  For x <= start sigma^2 = 0
 // and mix returns dst unchanged while teq is finite, so the blend is skipped
 // there; a non-finite free stream (sponge_all, uniform) blends every fluid
 // cell as the reference's mix does.
+// d in the step's units (g * unit); the free-stream state is scaled to them.
 template <bool FAST = false>
-__device__ __forceinline__ void sponge(uint32_t x, const StepParams& p, float (&d)[Q]) {
+__device__ __forceinline__ void sponge(uint32_t x, const StepParams& p, float (&d)[Q],
+                                       float unit = 1.0f) {
   const float dxs = (float)x - p.sponge_start;
   if (dxs > 0.0f || p.sponge_all) {
     const float sigma = fminf(fmaxf(dxs / p.sponge_len_m, 0.0f), 1.0f);
     const float sig2 = sigma * sigma;
 #pragma unroll
-    for (int i = 0; i < Q; ++i)
-      d[i] = FAST ? fmaf(sig2, p.tinf[i] - d[i], d[i]) : d[i] * (1.0f - sig2) + p.tinf[i] * sig2;
+    for (int i = 0; i < Q; ++i) {
+      const float ti = p.tinf[i] * unit;
+      d[i] = FAST ? fmaf(sig2, ti - d[i], d[i]) : d[i] * (1.0f - sig2) + ti * sig2;
+    }
   }
 }
 
 // The non-fluid branches of emitirPasso (ir.js:506-529): free-slip mirrors,
 // the inlet with its 1/7-power profile (wgsl.js:213-220) and the outlet.
 // Returns false for the fluid fall-through.
+// gi and d in the step's units (g * unit).
 __device__ __forceinline__ bool boundary_cell(uint32_t ct, const float (&gi)[Q], const Moments& M,
-                                              uint32_t z, const StepParams& p, float (&d)[Q]) {
+                                              uint32_t z, const StepParams& p, float (&d)[Q],
+                                              float unit = 1.0f) {
   if (ct == ESPELHO_Y) {
     // free-slip mirror (ir.js:383-390): reflect the y component
 #pragma unroll
@@ -601,21 +759,21 @@ __device__ __forceinline__ bool boundary_cell(uint32_t ct, const float (&gi)[Q],
     const float blf = zc < p.bl ? powf(zc / p.bl_m, 0.1428571429f) : 1.0f;
     const float ex = p.inlet[0] * blf, ey = p.inlet[1] * blf, ez = p.inlet[2] * blf;
     const float uuin = ex * ex + ey * ey + ez * ez;
-    d[0] = wf(0) * (0.0f - 1.5f * 1.0f * uuin);
+    d[0] = wf(0) * (0.0f - 1.5f * 1.0f * uuin) * unit;
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
       const float ci = dotc(i, ex, ey, ez);
-      d[i] = wf(i) * (0.0f + 1.0f * (3.0f * ci + 4.5f * ci * ci - 1.5f * uuin));
+      d[i] = wf(i) * (0.0f + 1.0f * (3.0f * ci + 4.5f * ci * ci - 1.5f * uuin)) * unit;
     }
     return true;
   }
   if (ct == SAIDA) {
     // outlet (ir.js:432-444): equilibrium at the local u with delta = 0
-    d[0] = wf(0) * (-1.5f * M.uu);
+    d[0] = wf(0) * (-1.5f * M.uu) * unit;
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
       const float cu = dotc(i, M.ux, M.uy, M.uz);
-      d[i] = wf(i) * (3.0f * cu + 4.5f * cu * cu - 1.5f * M.uu);
+      d[i] = wf(i) * (3.0f * cu + 4.5f * cu * cu - 1.5f * M.uu) * unit;
     }
     return true;
   }

@@ -119,36 +119,42 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
   const bool mixed = __ballot(!wall) != 0;
   if (wall) {
     if (MACROS) macros[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (!PULL_ONLY && mixed) {
-#pragma unroll
-      for (int i = 0; i < Q; ++i) out.st(i, own, 0.0f, 0u);
-    }
+    if (!PULL_ONLY && mixed) out.st_zero(own);
     return;
   }
   if (tb & NEAR_WALL) bounce_back<FORCE>(gi, sb, own, tipo, in, p, ct, f);
 
   // 2. moments, then the branch chain ESPELHO_Y -> ESPELHO_Z -> ENTRADA ->
   //    SAIDA -> fluid (equilibrium, Smagorinsky, TRT, sponge).
+  //    The arithmetic runs in the stored units (g * unit), see PopView.
   constexpr bool F = S::kFast;
-  const Moments M = moments<F>(gi);
+  constexpr float unit = unit_of<S>();
+  const Moments M = moments<F, S::kScaleLog2>(gi);
   if (PULL_ONLY) {
-    macros[cell] = make_float4(M.ux, M.uy, M.uz, M.delta);
+    macros[cell] = make_float4(M.ux, M.uy, M.uz, M.delta * (1.0f / unit));
     return;
   }
   float d[Q];
-  if (!boundary_cell(ct, gi, M, c.z, p, d)) {
-    float e[Q];
-    equilibrium<F>(M, e);
+  if (!boundary_cell(ct, gi, M, c.z, p, d, unit)) {
     float pab[6];
-    noneq_stress(gi, e, pab);
-    trt<F>(gi, e, smagorinsky_tau<F>(stress_norm<F>(pab), M.rho, p), p, d);
-    sponge<F>(c.x, p, d);
+    if constexpr (F) {
+      noneq_stress_fast(M, pab);
+      collide_fast(gi, M, smagorinsky_tau<F>(stress_norm<F>(pab), M.rho, p), p, d);
+    } else {
+      float e[Q];
+      equilibrium<F>(M, e);
+      noneq_stress(gi, e, pab);
+      trt<F>(gi, e, smagorinsky_tau<F>(stress_norm<F>(pab), M.rho, p), p, d);
+    }
+    sponge<F>(c.x, p, d, unit);
   }
 
   const uint32_t h = S::kStochastic ? sr_hash(cell, p.sr_seed) : 0u;
+  f16x2 r = {};
+  out.st_first(own, d[0], h, r);
 #pragma unroll
-  for (int i = 0; i < Q; ++i) out.st(i, own, d[i], h);
-  if (MACROS) macros[cell] = make_float4(M.ux, M.uy, M.uz, M.delta);
+  for (int i = 1; i < Q; i += 2) out.st_pair(i, own, d[i], d[i + 1], h, r);
+  if (MACROS) macros[cell] = make_float4(M.ux, M.uy, M.uz, M.delta * (1.0f / unit));
 }
 
 template <class S, bool MACROS, int BLOCK, int LB>

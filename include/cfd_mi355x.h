@@ -73,7 +73,8 @@ typedef struct cfd_params {
  * picks the shape measured fastest for the set: float32 sets above 256 MB
  * get 512 threads with the start rotated by 36 % of the grid, float16 sets
  * above 128 MB and float32 sets of 128-256 MB alternate the order, others
- * run 256 threads in ascending order; all with XCD chunks of 8. */
+ * run 256 threads in ascending order; all with XCD chunks of 8.  Float16
+ * sets up to 128 MB are SoA, every other set below 4 GiB blocked. */
 typedef struct cfd_tuning {
   int32_t block;        /* threads per workgroup: 256 or 512                   */
   int32_t remap;        /* 0/1: launch order; K (power of two): each XCD steps

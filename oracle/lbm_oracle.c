@@ -78,7 +78,11 @@ static void tables(void) {
  * features, the weights and each hidden activation are rounded to bf16
  * (round to nearest even); products accumulate in float32 from the bias in
  * increasing input order (the matrix cores sum in another order, so results
- * agree to float32 accumulation error, not bit for bit).  Returns
+ * agree to float32 accumulation error, not bit for bit).  The network is
+ * evaluated in its base-2 form, as the matrix-core path: W1, b1 and b2 are
+ * multiplied by -log2(e) in float32 (before the bf16 rounding of W1), so
+ * layers 1 and 2 give u = -log2(e) z and the SiLU is u / (1 + 2^u) =
+ * -log2(e) SiLU(z); W2 is unchanged and w3 is multiplied by -ln 2.  Returns
  * out_scale * softplus(z), or NaN when a normalized feature leaves
  * [lo, hi] (the caller then keeps the Smagorinsky tau). */
 typedef struct {
@@ -96,7 +100,8 @@ static float bf16r(float f) {
   return f;
 }
 
-static float silu_ref(float a) { return a * (1.0f / (1.0f + expf(-a))); }
+static const float kNegLog2e = -1.44269504f, kNegLn2 = -0.693147181f;
+static float silu2_ref(float u) { return u * (1.0f / (1.0f + exp2f(u))); }
 
 static float orc_closure_nu(const orc_closure* c, const float x[8]) {
   const float *w1 = c->p, *b1 = c->p + 256, *w2 = c->p + 288, *b2 = c->p + 1312,
@@ -108,17 +113,17 @@ static float orc_closure_nu(const orc_closure* c, const float x[8]) {
     xb[k] = bf16r(xn);
   }
   for (int o = 0; o < 32; ++o) {
-    float acc = b1[o];
-    for (int k = 0; k < 8; ++k) acc += bf16r(w1[o * 8 + k]) * xb[k];
-    h1[o] = bf16r(silu_ref(acc));
+    float acc = kNegLog2e * b1[o];
+    for (int k = 0; k < 8; ++k) acc += bf16r(kNegLog2e * w1[o * 8 + k]) * xb[k];
+    h1[o] = bf16r(silu2_ref(acc));
   }
   for (int o = 0; o < 32; ++o) {
-    float acc = b2[o];
+    float acc = kNegLog2e * b2[o];
     for (int k = 0; k < 32; ++k) acc += bf16r(w2[o * 32 + k]) * h1[k];
-    h2[o] = bf16r(silu_ref(acc));
+    h2[o] = bf16r(silu2_ref(acc));
   }
   float z = b3;
-  for (int k = 0; k < 32; ++k) z += bf16r(w3[k]) * h2[k];
+  for (int k = 0; k < 32; ++k) z += bf16r(kNegLn2 * w3[k]) * h2[k];
   const float sp = z > 20.0f ? z : (z < -15.0f ? expf(z) : log1pf(expf(z)));
   return c->out_scale * sp;
 }

@@ -351,20 +351,51 @@ def _bf16(a):
     return u.astype(np.uint32).view(np.float32)
 
 
-def _mlp_numpy(p, xn):
-    """Independent float64 evaluation of the closure MLP on bf16-rounded
-    weights, inputs and hidden activations (the oracle sums in float32)."""
-    sig = lambda a: 1.0 / (1.0 + np.exp(-a))
-    w1 = _bf16(p[0:256]).reshape(32, 8).astype(np.float64)
-    w2 = _bf16(p[288:1312]).reshape(32, 32).astype(np.float64)
-    w3 = _bf16(p[1344:1376]).astype(np.float64)
-    x = _bf16(xn).astype(np.float64)
-    a1 = x @ w1.T + p[256:288]
-    h1 = _bf16((a1 * sig(a1)).astype(np.float32)).astype(np.float64)
-    a2 = h1 @ w2.T + p[1312:1344]
-    h2 = _bf16((a2 * sig(a2)).astype(np.float32)).astype(np.float64)
+NL2E, NLN2 = np.float32(-1.44269504), np.float32(-0.693147181)
+
+
+def _mlp_numpy(p, xn, bf16=True):
+    """Independent float64 evaluation of the closure MLP in its base-2 form:
+    W1, b1 and b2 times -log2(e) and w3 times -ln 2, each product in float32;
+    with bf16, weights, inputs and hidden activations rounded to bf16 (the
+    oracle sums in float32).  SiLU2(u) = u / (1 + 2^u)."""
+    r = _bf16 if bf16 else (lambda a: np.asarray(a, np.float32))
+    silu2 = lambda u: u / (1.0 + np.exp2(u))
+    w1 = r(NL2E * p[0:256]).reshape(32, 8).astype(np.float64)
+    w2 = r(p[288:1312]).reshape(32, 32).astype(np.float64)
+    w3 = r(NLN2 * p[1344:1376]).astype(np.float64)
+    x = r(xn).astype(np.float64)
+    a1 = x @ w1.T + (NL2E * p[256:288]).astype(np.float64)
+    h1 = r(silu2(a1).astype(np.float32)).astype(np.float64)
+    a2 = h1 @ w2.T + (NL2E * p[1312:1344]).astype(np.float64)
+    h2 = r(silu2(a2).astype(np.float32)).astype(np.float64)
     z = h2 @ w3 + p[1376]
     return np.logaddexp(0.0, z)
+
+
+def _mlp_plain64(p, xn):
+    """The same network in its usual form, float64 throughout (no rounding)."""
+    p = p.astype(np.float64)
+    silu = lambda a: a / (1.0 + np.exp(-a))
+    h1 = silu(xn.astype(np.float64) @ p[0:256].reshape(32, 8).T + p[256:288])
+    h2 = silu(h1 @ p[288:1312].reshape(32, 32).T + p[1312:1344])
+    return np.logaddexp(0.0, h2 @ p[1344:1376] + p[1376])
+
+
+def test_base2_form_is_the_same_network():
+    """-log2(e) SiLU(z) = u / (1 + 2^u) at u = -log2(e) z, and w3 (-ln 2)
+    undoes the factor: without the bf16 roundings the base-2 evaluation is
+    the usual network to float32 rounding of the scaled weights."""
+    rng = np.random.default_rng(5)
+    p = rng.normal(0, 0.4, 1377).astype(np.float32)
+    xn = rng.normal(0, 1.5, (500, 8)).astype(np.float32)
+    a = _mlp_numpy(p, xn, bf16=False)
+    b = _mlp_plain64(p, xn)
+    assert np.abs(a - b).max() < 1e-5 * np.abs(b).max()
+    # a wrong constant or sign in the transform would not come close
+    q = p.copy()
+    q[1344:1376] = -q[1344:1376]
+    assert np.abs(_mlp_plain64(q, xn) - a).max() > 1e-2 * np.abs(b).max()
 
 
 def test_oracle_closure_mlp_against_independent_numpy(orc):
