@@ -719,17 +719,28 @@ __device__ __forceinline__ void collide_fast(const float (&gi)[Q], const Moments
 // there; a non-finite free stream (sponge_all, uniform) blends every fluid
 // cell as the reference's mix does.
 // d in the step's units (g * unit); the free-stream state is scaled to them.
+// FAST: sigma from one reciprocal, clamped by a median, and the blend as
+// d + sigma^2 (unit tinf - d) = fma(sigma^2 unit, tinf, fma(-sigma^2, d, d)),
+// two instructions per population with tinf read as a scalar operand.
 template <bool FAST = false>
 __device__ __forceinline__ void sponge(uint32_t x, const StepParams& p, float (&d)[Q],
                                        float unit = 1.0f) {
   const float dxs = (float)x - p.sponge_start;
   if (dxs > 0.0f || p.sponge_all) {
-    const float sigma = fminf(fmaxf(dxs / p.sponge_len_m, 0.0f), 1.0f);
-    const float sig2 = sigma * sigma;
+    if constexpr (FAST) {
+      const float sigma =
+          __builtin_amdgcn_fmed3f(dxs * __builtin_amdgcn_rcpf(p.sponge_len_m), 0.0f, 1.0f);
+      const float sig2 = sigma * sigma, sig2u = sig2 * unit;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const float ti = p.tinf[i] * unit;
-      d[i] = FAST ? fmaf(sig2, ti - d[i], d[i]) : d[i] * (1.0f - sig2) + ti * sig2;
+      for (int i = 0; i < Q; ++i) d[i] = fmaf(sig2u, p.tinf[i], fmaf(-sig2, d[i], d[i]));
+    } else {
+      const float sigma = fminf(fmaxf(dxs / p.sponge_len_m, 0.0f), 1.0f);
+      const float sig2 = sigma * sigma;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const float ti = p.tinf[i] * unit;
+        d[i] = d[i] * (1.0f - sig2) + ti * sig2;
+      }
     }
   }
 }
