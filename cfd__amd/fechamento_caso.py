@@ -83,12 +83,15 @@ def _units():
 
 def rodar(caso: dict, passos: int, precisao: str = "fp32", fech: Fechamento | None = None,
           amostra: int = 100, media_desde: int | None = None, instantaneos=(), ao_instantaneo=None,
-          campo_cada: int = 0, log=print) -> dict:
-    """One tunnel run: ramp over 1000 coarse-equivalent steps, then `passos`
-    steps; forces every `amostra` steps; Cd / Cl / St over steps >=
-    media_desde; `ao_instantaneo(solver, k)` at the listed steps; with
-    campo_cada > 0 also the time-averaged velocity field over the same window,
-    sampled every campo_cada steps ('campo', float32 (nz, ny, nx, 3))."""
+          campo_cada: int = 0, log=print, rampa_extra: int = 0) -> dict:
+    """One tunnel run: ramp over 1000 coarse-equivalent steps (plus
+    rampa_extra steps of this lattice), then `passos` steps; forces every
+    `amostra` steps; Cd / Cl / St over steps >= media_desde;
+    `ao_instantaneo(solver, k)` at the listed steps; with campo_cada > 0 also
+    the time-averaged velocity field over the same window, sampled every
+    campo_cada steps ('campo', float32 (nz, ny, nx, 3)).  A different
+    rampa_extra gives another realization of an unsteady flow with the same
+    statistics (the ramp ends long before the averaging window)."""
     from .solver import SolverMI355X
 
     g, fator = caso["grade"], caso["fator"]
@@ -100,7 +103,8 @@ def rodar(caso: dict, passos: int, precisao: str = "fp32", fech: Fechamento | No
         s.definirTipos(caso["tipo"])
         s.configurar(inletU=[U, 0.0, 0.0], **cfg)
         s.calibrarRepouso()
-        s.rampa(1000 * fator, lambda f: s.configurar(inletU=[U * f, 0.0, 0.0], **cfg))
+        s.rampa(1000 * fator + rampa_extra,
+                lambda f: s.configurar(inletU=[U * f, 0.0, 0.0], **cfg))
         s.inicializar()
         if fech is not None:
             s.calcularDistanciaParede()
@@ -341,8 +345,18 @@ def calibrar(fech: Fechamento, seeds=TREINO_SEEDS[:3], fatores=(1.0, 10.0, 100.0
 
 # ------------------------------------------------------------------ judge
 
+# Ramp-length offsets of the realizations of an ensemble judging (coarse steps).
+RAMPA_CONJUNTO = (0, 37, 74, 111, 148, 185, 222, 259)
+
+
 def avaliar(fech: Fechamento, seed: int = AVALIACAO_SEED, passos_grossos: int = 9000,
-            log=print) -> dict:
+            log=print, conjunto: int = 1) -> dict:
+    """The fine run against the four coarse variants.  conjunto > 1: each
+    coarse variant also runs as an ensemble of that many realizations (ramp
+    offsets RAMPA_CONJUNTO), and the summary adds the ensemble-mean Cd error
+    with its standard error: the windowed Cd of one realization of these
+    unsteady coarse wakes scatters by several per cent."""
+    assert 1 <= conjunto <= len(RAMPA_CONJUNTO)
     fino, grosso = montar(seed, 2), montar(seed, 1)
     out = {}
     desde = passos_grossos // 3
@@ -359,6 +373,16 @@ def avaliar(fech: Fechamento, seed: int = AVALIACAO_SEED, passos_grossos: int = 
                                                  out["fino_fp32_smagorinsky"]["campo"], grosso)
         log(f"[avaliar] {nome}: Cd {out[nome]['cd']:.4f}, wake-field error "
             f"{out[nome]['erro_esteira']:.5f}, {out[nome]['mlups']:.0f} MLUPS")
+        cds = [out[nome]["cd"]]
+        for extra in RAMPA_CONJUNTO[1:conjunto]:
+            r = rodar(grosso, passos_grossos, prec, fech=f, amostra=100, media_desde=desde,
+                      rampa_extra=extra)
+            cds.append(r["cd"])
+        out[nome]["cd_conjunto"] = cds
+        if conjunto > 1:
+            log(f"[avaliar] {nome}: ensemble Cd {np.mean(cds):.4f} +- "
+                f"{np.std(cds, ddof=1) / math.sqrt(len(cds)):.4f} ({len(cds)} realizations: "
+                + ", ".join(f"{c:.4f}" for c in cds) + ")")
     cd_f = out["fino_fp32_smagorinsky"]["cd"]
     st_f = out["fino_fp32_smagorinsky"]["st_comprimento"]["lateral"]
     err = {k: (v["cd"] - cd_f) / cd_f for k, v in out.items() if k.startswith("grosso")}
@@ -374,6 +398,12 @@ def avaliar(fech: Fechamento, seed: int = AVALIACAO_SEED, passos_grossos: int = 
         "ganho_vs_fino": ganho,
         "juiz": "aprendido" if aceito else "smagorinsky (o aprendido ficou mais longe do fino)",
         "erro_esteira": {k: v["erro_esteira"] for k, v in out.items() if k.startswith("grosso")},
+        "erro_cd_conjunto": {k: (float(np.mean(v["cd_conjunto"])) - cd_f) / cd_f
+                             for k, v in out.items() if k.startswith("grosso")},
+        "erro_padrao_conjunto": {
+            k: (float(np.std(v["cd_conjunto"], ddof=1) / math.sqrt(len(v["cd_conjunto"])) / cd_f)
+                if len(v["cd_conjunto"]) > 1 else None)
+            for k, v in out.items() if k.startswith("grosso")},
     }
     for v in out.values():
         if isinstance(v, dict):
@@ -401,6 +431,9 @@ def main(argv=None):
     ap.add_argument("--sementes", default=None,
                     help="with --carregar: judge on these held-out vehicle seeds (e.g. 7,8,9) "
                          "and summarize the Cd errors over them")
+    ap.add_argument("--conjunto", type=int, default=1,
+                    help="with --sementes: judge each coarse variant also as the mean of this "
+                         "many realizations (inflow ramps of different lengths)")
     a = ap.parse_args(argv)
     if a.perfil:
         # the coarse float16 closure run alone (kernel profiles), plus the copy
@@ -421,11 +454,16 @@ def main(argv=None):
             if a.modelo:
                 fech.salvar(a.modelo)
         for sd in sementes:
-            r = avaliar(fech, seed=sd, passos_grossos=a.passos_grossos)
+            r = avaliar(fech, seed=sd, passos_grossos=a.passos_grossos, conjunto=a.conjunto)
             res["por_semente"][sd] = r["resumo"]
             print(f"[sementes] {sd}: fine Cd {r['resumo']['cd_fino']:.4f}, errors "
                   + ", ".join(f"{k} {v:+.3f}" for k, v in r["resumo"]["erro_cd"].items()),
                   flush=True)
+            if a.conjunto > 1:
+                print(f"[sementes] {sd}: ensemble-mean errors "
+                      + ", ".join(f"{k} {v:+.3f} +- {r['resumo']['erro_padrao_conjunto'][k]:.3f}"
+                                  for k, v in r["resumo"]["erro_cd_conjunto"].items()),
+                      flush=True)
         variantes = next(iter(res["por_semente"].values()))["erro_cd"].keys()
         res["resumo"] = {
             v: {"erro_abs_medio": float(np.mean([abs(r["erro_cd"][v])
@@ -435,8 +473,13 @@ def main(argv=None):
                 "dentro_3pct": int(sum(abs(r["erro_cd"][v]) <= 0.03
                                        for r in res["por_semente"].values())),
                 "erro_esteira_medio": float(np.mean([r["erro_esteira"][v]
-                                                     for r in res["por_semente"].values()]))}
+                                                     for r in res["por_semente"].values()])),
+                "conjunto_erro_abs_medio": float(np.mean([abs(r["erro_cd_conjunto"][v])
+                                                          for r in res["por_semente"].values()])),
+                "conjunto_dentro_3pct": int(sum(abs(r["erro_cd_conjunto"][v]) <= 0.03
+                                                for r in res["por_semente"].values()))}
             for v in variantes}
+        res["resumo"]["conjunto"] = a.conjunto
         res["resumo"]["sementes"] = sementes
         if a.saida:
             with open(a.saida, "w") as fh:
