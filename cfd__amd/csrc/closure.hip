@@ -213,7 +213,7 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   float d[Q];
   if (!boundary_cell(ct, gi, M, c.z, p, d, unit)) {
     if constexpr (F) {
-      collide_fast(gi, M, tau, p, d);
+      collide_fast<(S::kScaleLog2 > 0)>(gi, M, tau, p, d);
     } else {
       float e[Q];
       equilibrium<F>(M, e);

@@ -685,6 +685,14 @@ __device__ __forceinline__ void trt(const float (&gi)[Q], const float (&e)[Q], f
 // e_i + e_j = w (2 delta + rho (9 cu^2 - 3 uu)) and difference
 // e_i - e_j = 6 w rho cu, so no equilibrium array is formed.  Then, as trt<true>,
 // g -/+ (omega+/2) (sum_g - sum_e) -/+ (omega-/2) (dif_g - dif_e).
+//
+// EXPANDED (the float16 steps): the same update with the g terms gathered,
+// d_i = A g_i + B g_j + (P + M), d_j = A g_j + B g_i + (P - M), where
+// A = 1 - omega+/2 - omega-/2, B = omega-/2 - omega+/2, P = (omega+/2)(e_i + e_j)
+// and M = (omega-/2)(e_i - e_j): two instructions fewer per pair.  The terms
+// then cancel to the size of the result (a few float32 ulp of it, against
+// the 2^-11 of float16 storage), so the float32 steps keep the form above.
+template <bool EXPANDED = false>
 __device__ __forceinline__ void collide_fast(const float (&gi)[Q], const Moments& M, float tau,
                                              const StepParams& p, float (&d)[Q]) {
   const float omP = __builtin_amdgcn_rcpf(tau);
@@ -695,6 +703,24 @@ __device__ __forceinline__ void collide_fast(const float (&gi)[Q], const Moments
   d[0] = fmaf(-omP, gi[0] - e0, gi[0]);
   // t = 2 delta - 3 rho uu; per weight class A = w t, B = 9 w rho, C = 6 w rho
   const float t = fmaf(-3.0f * M.rho, M.uu, 2.0f * M.delta);
+  if constexpr (EXPANDED) {
+    const float A = (1.0f - hp) - hm, B = hm - hp;
+    const float hpt = hp * t, hpr = hp * M.rho, hmr = hm * M.rho;
+    const float HAa = wf(1) * hpt, HAd = wf(7) * hpt;
+    const float HBa = (9.0f * wf(1)) * hpr, HBd = (9.0f * wf(7)) * hpr;
+    const float HCa = (6.0f * wf(1)) * hmr, HCd = (6.0f * wf(7)) * hmr;
+#pragma unroll
+    for (int i = 1; i < Q; i += 2) {
+      const int j = i + 1;
+      const bool axis = i <= 6;
+      const float cu = dotc(i, M.ux, M.uy, M.uz);
+      const float P = fmaf(axis ? HBa : HBd, cu * cu, axis ? HAa : HAd);
+      const float Mm = (axis ? HCa : HCd) * cu;
+      d[i] = fmaf(A, gi[i], fmaf(B, gi[j], P + Mm));
+      d[j] = fmaf(A, gi[j], fmaf(B, gi[i], P - Mm));
+    }
+    return;
+  }
   const float Aa = wf(1) * t, Ad = wf(7) * t;
   const float Ba = (9.0f * wf(1)) * M.rho, Bd = (9.0f * wf(7)) * M.rho;
   const float Ca = (6.0f * wf(1)) * M.rho, Cd = (6.0f * wf(7)) * M.rho;

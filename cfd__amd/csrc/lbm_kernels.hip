@@ -139,7 +139,7 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
     float pab[6];
     if constexpr (F) {
       noneq_stress_fast(M, pab);
-      collide_fast(gi, M, smagorinsky_tau<F>(stress_norm<F>(pab), M.rho, p), p, d);
+      collide_fast<(S::kScaleLog2 > 0)>(gi, M, smagorinsky_tau<F>(stress_norm<F>(pab), M.rho, p), p, d);
     } else {
       float e[Q];
       equilibrium<F>(M, e);
