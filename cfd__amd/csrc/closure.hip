@@ -164,7 +164,7 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
     // the exact form's equilibrium is recomputed after the MLP rather than
     // held across it (fewer registers at the matrix instructions)
     float e[Q];
-    equilibrium<F>(M, e);
+    equilibrium(M, e);
     noneq_stress(gi, e, pab);
   }
   const float qs = stress_norm<F>(pab);
@@ -216,8 +216,8 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
       collide_fast<(S::kScaleLog2 > 0)>(gi, M, tau, p, d);
     } else {
       float e[Q];
-      equilibrium<F>(M, e);
-      trt<F>(gi, e, tau, p, d);
+      equilibrium(M, e);
+      trt(gi, e, tau, p, d);
     }
     sponge<F>(c.x, p, d, unit);
   }

@@ -62,14 +62,10 @@ struct StoreF32 {
   static __device__ __forceinline__ float ld(P p) { return *p; }
   template <class P>
   static __device__ __forceinline__ void st(P p, float v) { *p = v; }
-  static __device__ __forceinline__ float dec(Bits b) { return __uint_as_float(b); }
-  static __device__ __forceinline__ Bits enc(float v) { return __float_as_uint(v); }
   // the step's arithmetic runs in the stored units: value * 2^kScaleLog2
   static constexpr int kScaleLog2 = 0;
   template <class P>
   static __device__ __forceinline__ float ld_s(P p) { return *p; }
-  template <class P>
-  static __device__ __forceinline__ void st_s(P p, float v) { *p = v; }
   static __device__ __forceinline__ float dec_s(Bits b) { return __uint_as_float(b); }
   static __device__ __forceinline__ Bits enc_s(float v) { return __float_as_uint(v); }
 };
@@ -92,12 +88,6 @@ struct StoreF16 {
   template <class P>
   static __device__ __forceinline__ void st(P p, float v) { *p = static_cast<T>(v * 4096.0f); }
   using Bits = uint16_t;
-  static __device__ __forceinline__ float dec(Bits b) {
-    return static_cast<float>(__builtin_bit_cast(T, b)) * 0.000244140625f;
-  }
-  static __device__ __forceinline__ Bits enc(float v) {
-    return __builtin_bit_cast(Bits, static_cast<T>(v * 4096.0f));
-  }
   // The step keeps the populations in the stored units (g * 2^12) through
   // its arithmetic: scaling by a power of two commutes with every rounding,
   // so the collision rounds exactly as it would on g itself, without a
@@ -105,8 +95,6 @@ struct StoreF16 {
   static constexpr int kScaleLog2 = 12;
   template <class P>
   static __device__ __forceinline__ float ld_s(P p) { return static_cast<float>(*p); }
-  template <class P>
-  static __device__ __forceinline__ void st_s(P p, float v) { *p = static_cast<T>(v); }
   static __device__ __forceinline__ float dec_s(Bits b) {
     return static_cast<float>(__builtin_bit_cast(T, b));
   }
@@ -553,26 +541,14 @@ __device__ __forceinline__ Moments moments(const float (&gi)[Q]) {
   }
 }
 
-// Shifted equilibrium (ir.js:216-232).  FAST: the same polynomial as
-// w (delta + rho (cu (3 + 4.5 cu) - 1.5 uu)) with fused multiply-adds.
-template <bool FAST = false>
+// Shifted equilibrium (ir.js:216-232), the exact form's (the fast form never
+// forms the 19 equilibria: noneq_stress_fast and collide_fast).
 __device__ __forceinline__ void equilibrium(const Moments& M, float (&e)[Q]) {
-  if constexpr (FAST) {
-    const float u15 = 1.5f * M.uu;
-    e[0] = wf(0) * fmaf(-M.rho, u15, M.delta);
+  e[0] = wf(0) * (M.delta - 1.5f * M.rho * M.uu);
 #pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      const float cu = dotc(i, M.ux, M.uy, M.uz);
-      const float q = fmaf(cu, fmaf(4.5f, cu, 3.0f), -u15);
-      e[i] = fmaf(wf(i) * M.rho, q, wf(i) * M.delta);
-    }
-  } else {
-    e[0] = wf(0) * (M.delta - 1.5f * M.rho * M.uu);
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      const float cu = dotc(i, M.ux, M.uy, M.uz);
-      e[i] = wf(i) * (M.delta + M.rho * (3.0f * cu + 4.5f * cu * cu - 1.5f * M.uu));
-    }
+  for (int i = 1; i < Q; ++i) {
+    const float cu = dotc(i, M.ux, M.uy, M.uz);
+    e[i] = wf(i) * (M.delta + M.rho * (3.0f * cu + 4.5f * cu * cu - 1.5f * M.uu));
   }
 }
 
@@ -645,28 +621,9 @@ __device__ __forceinline__ float smagorinsky_tau(float qs, float rho, const Step
 }
 
 // TRT collision per opposite pair with omega- from the fixed magic
-// parameter (ir.js:298-323).
-template <bool FAST = false>
+// parameter (ir.js:298-323), the exact form's.
 __device__ __forceinline__ void trt(const float (&gi)[Q], const float (&e)[Q], float tau,
                                     const StepParams& p, float (&d)[Q]) {
-  if constexpr (FAST) {
-    // omega+ = 1/tau; omega- = 1 / (Lambda / (tau - 1/2) + 1/2); per pair
-    // g -/+ (omega+/2) (sum_g - sum_e) -/+ (omega-/2) (dif_g - dif_e)
-    const float omP = __builtin_amdgcn_rcpf(tau);
-    const float omM = __builtin_amdgcn_rcpf(
-        fmaf(p.magic, __builtin_amdgcn_rcpf(tau - 0.5f), 0.5f));
-    const float hp = 0.5f * omP, hm = 0.5f * omM;
-    d[0] = fmaf(-omP, gi[0] - e[0], gi[0]);
-#pragma unroll
-    for (int i = 1; i < Q; i += 2) {
-      const int j = i + 1;
-      const float sp = (gi[i] + gi[j]) - (e[i] + e[j]);
-      const float sm = (gi[i] - gi[j]) - (e[i] - e[j]);
-      d[i] = fmaf(-hm, sm, fmaf(-hp, sp, gi[i]));
-      d[j] = fmaf(hm, sm, fmaf(-hp, sp, gi[j]));
-    }
-    return;
-  }
   const float omP = 1.0f / tau;
   const float omM = 1.0f / (p.magic / (1.0f / omP - 0.5f) + 0.5f);
   d[0] = gi[0] - omP * (gi[0] - e[0]);
@@ -683,7 +640,8 @@ __device__ __forceinline__ void trt(const float (&gi)[Q], const float (&e)[Q], f
 // FAST TRT (ir.js:298-323) from the moments, per opposite pair (i, j = i + 1)
 // with cu = c_i . u (c_j = -c_i): the equilibrium's pair sum
 // e_i + e_j = w (2 delta + rho (9 cu^2 - 3 uu)) and difference
-// e_i - e_j = 6 w rho cu, so no equilibrium array is formed.  Then, as trt<true>,
+// e_i - e_j = 6 w rho cu, so no equilibrium array is formed.  Then, with
+// 1-ulp reciprocals for omega+ = 1/tau and omega- = 1 / (Lambda / (tau - 1/2) + 1/2),
 // g -/+ (omega+/2) (sum_g - sum_e) -/+ (omega-/2) (dif_g - dif_e).
 //
 // EXPANDED (the float16 steps): the same update with the g terms gathered,

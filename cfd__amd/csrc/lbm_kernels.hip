@@ -142,9 +142,9 @@ __device__ __forceinline__ void step_cell(uint32_t cell,
       collide_fast<(S::kScaleLog2 > 0)>(gi, M, smagorinsky_tau<F>(stress_norm<F>(pab), M.rho, p), p, d);
     } else {
       float e[Q];
-      equilibrium<F>(M, e);
+      equilibrium(M, e);
       noneq_stress(gi, e, pab);
-      trt<F>(gi, e, smagorinsky_tau<F>(stress_norm<F>(pab), M.rho, p), p, d);
+      trt(gi, e, smagorinsky_tau<F>(stress_norm<F>(pab), M.rho, p), p, d);
     }
     sponge<F>(c.x, p, d, unit);
   }
