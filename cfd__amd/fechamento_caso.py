@@ -311,43 +311,52 @@ def calibrar_esteira(fech: Fechamento, seeds=TREINO_SEEDS[:3],
             "fator": melhor["fator"], "out_scale": fech.out_scale}
 
 
+# Ramp-length offsets of the realizations of an ensemble (coarse steps).
+RAMPA_CONJUNTO = (0, 37, 74, 111, 148, 185, 222, 259)
+
+
 def calibrar(fech: Fechamento, seeds=TREINO_SEEDS[:3], fatores=(1.0, 10.0, 100.0, 1000.0),
-             passos_grossos: int = 9000, log=print) -> dict:
+             passos_grossos: int = 9000, log=print, conjunto: int = 1) -> dict:
     """Solver in the loop: scale the trained output by each factor, run the
     coarse float16 tunnel of each calibration shape and keep the factor whose
     time-averaged Cd is closest to the fine runs' (mean relative error).  The
     a-priori target only sees the sub-grid stress of the filtered field; the
     coarse run also needs the dissipation that its own discretization lacks,
-    which only a coarse run can measure."""
+    which only a coarse run can measure.  conjunto > 1: each coarse Cd is the
+    mean of that many realizations (ramp offsets RAMPA_CONJUNTO), since one
+    realization's windowed Cd scatters by several per cent."""
+    assert 1 <= conjunto <= len(RAMPA_CONJUNTO)
     finos, grossos = {}, {}
     for sd in seeds:
         grossos[sd] = montar(sd, 1)
         finos[sd] = rodar(montar(sd, 2), 2 * passos_grossos, "fp32", amostra=200,
                           media_desde=2 * passos_grossos // 3)["cd"]
+        log(f"[calibrar] fine Cd of shape {sd}: {finos[sd]:.4f}")
+
+    def cd_grosso(sd, f):
+        return float(np.mean([rodar(grossos[sd], passos_grossos, "fp16sr", fech=f,
+                                    media_desde=passos_grossos // 3, rampa_extra=extra)["cd"]
+                              for extra in RAMPA_CONJUNTO[:conjunto]]))
+
     medidas = []
     base = fech.out_scale
     for fa in fatores:
         fech.out_scale = base * fa
-        cds = [rodar(grossos[sd], passos_grossos, "fp16sr", fech=fech,
-                     media_desde=passos_grossos // 3)["cd"] for sd in seeds]
+        cds = [cd_grosso(sd, fech) for sd in seeds]
         err = float(np.mean([abs(c - finos[sd]) / finos[sd] for c, sd in zip(cds, seeds)]))
         medidas.append({"fator": fa, "cd": cds, "erro_medio": err})
-        log(f"[calibrar] factor {fa:g}: mean |Cd error| {err:.4f}")
-    smag = [rodar(grossos[sd], passos_grossos, "fp16sr", media_desde=passos_grossos // 3)["cd"]
-            for sd in seeds]
+        log(f"[calibrar] factor {fa:g}: mean |Cd error| {err:.4f} (Cd "
+            + ", ".join(f"{c:.4f}" for c in cds) + ")")
+    smag = [cd_grosso(sd, None) for sd in seeds]
     err_smag = float(np.mean([abs(c - finos[sd]) / finos[sd] for c, sd in zip(smag, seeds)]))
     melhor = min(medidas, key=lambda m: m["erro_medio"])
     fech.out_scale = base * melhor["fator"]
     log(f"[calibrar] chosen factor {melhor['fator']:g} (Smagorinsky: {err_smag:.4f})")
     return {"finos": finos, "medidas": medidas, "smagorinsky": {"cd": smag, "erro_medio": err_smag},
-            "fator": melhor["fator"]}
+            "fator": melhor["fator"], "conjunto": conjunto}
 
 
 # ------------------------------------------------------------------ judge
-
-# Ramp-length offsets of the realizations of an ensemble judging (coarse steps).
-RAMPA_CONJUNTO = (0, 37, 74, 111, 148, 185, 222, 259)
-
 
 def avaliar(fech: Fechamento, seed: int = AVALIACAO_SEED, passos_grossos: int = 9000,
             log=print, conjunto: int = 1) -> dict:
@@ -449,6 +458,12 @@ def main(argv=None):
         sementes = [int(x) for x in a.sementes.split(",")]
         assert not set(sementes) & set(TREINO_SEEDS), "judging seeds must be held out"
         res = {"por_semente": {}}
+        if a.calibrar:
+            res["calibracao"] = calibrar(fech, passos_grossos=a.passos_grossos,
+                                         fatores=(0.25, 0.5, 1.0, 2.0, 4.0, 8.0),
+                                         conjunto=a.conjunto)
+            if a.modelo:
+                fech.salvar(a.modelo)
         if a.calibrar_esteira:
             res["calibracao_esteira"] = calibrar_esteira(fech, passos_grossos=a.passos_grossos)
             if a.modelo:

@@ -446,3 +446,46 @@ def test_oracle_closure_step_reduces_to_the_plain_step(orc):
                                      np.full(8, np.inf), 0.0)
             b, _ = orc.step_closure(grid, g0, tipo, par, zero, wd)
             assert np.array_equal(b, plain)
+
+
+# ---- the identities the step's fast form rests on (float64, exact tables)
+
+def _eq_shifted(delta, u):
+    """Shifted equilibrium e_i = w_i (delta + rho (3 cu + 4.5 cu^2 - 1.5 uu))."""
+    rho = 1.0 + delta
+    cu = C @ u
+    return W * (delta + rho * (3.0 * cu + 4.5 * cu * cu - 1.5 * (u @ u)))
+
+
+def test_fast_form_identities():
+    """noneq_stress_fast takes Pi = S - m u - delta/3 I from the second
+    moment of the shifted equilibrium, sum c_a c_b e = delta/3 I + rho u u
+    (fourth-order isotropy of the weights); collide_fast takes the pair sums
+    e_i + e_j = w (2 delta + rho (9 cu^2 - 3 uu)) and e_i - e_j = 6 w rho cu
+    for opposite pairs (i, i + 1).  A wrong weight class, sign or factor
+    fails one of them."""
+    rng = np.random.default_rng(11)
+    for _ in range(50):
+        delta = rng.normal(0, 0.05)
+        u = rng.normal(0, 0.1, 3)
+        rho = 1.0 + delta
+        e = _eq_shifted(delta, u)
+        second = np.einsum("i,ia,ib->ab", e, C.astype(np.float64), C.astype(np.float64))
+        want = delta / 3.0 * np.eye(3) + rho * np.outer(u, u)
+        assert np.abs(second - want).max() < 1e-15
+        # Pi of an arbitrary state from its moments equals sum c c (g - e)
+        g = e + rng.normal(0, 1e-3, Q) * W
+        d = g.sum()
+        m = C.T.astype(np.float64) @ g
+        ug = m / (1.0 + d)
+        eg = _eq_shifted(d, ug)
+        pi_direct = np.einsum("i,ia,ib->ab", g - eg, C.astype(np.float64), C.astype(np.float64))
+        S = np.einsum("i,ia,ib->ab", g, C.astype(np.float64), C.astype(np.float64))
+        pi_moments = S - np.outer(m, ug) - d / 3.0 * np.eye(3)
+        assert np.abs(pi_direct - pi_moments).max() < 1e-15
+        for i in range(1, Q, 2):
+            j = i + 1
+            assert OPP[i] == j and np.array_equal(C[j], -C[i])
+            cu = C[i] @ u
+            assert abs((e[i] + e[j]) - W[i] * (2 * delta + rho * (9 * cu * cu - 3 * (u @ u)))) < 1e-16
+            assert abs((e[i] - e[j]) - 6 * W[i] * rho * cu) < 1e-16
