@@ -200,7 +200,10 @@ void cfd_destroy(cfd_solver *s);
  * whose normalized features leave [in_lo, in_hi], or whose output is not
  * finite, keeps the Smagorinsky tau (counted by cfd_closure_fallbacks).
  * Inference uses bf16 operands with float32 accumulation on the matrix
- * cores; training (cfd_mlp_train) is float32 Adam on minibatches. */
+ * cores, the network evaluated in base 2 (w1, b1, b2 times -log2(e) and w3
+ * times -ln 2 in float32, then bf16: the same function, with the bf16
+ * roundings of that form); training (cfd_mlp_train) is float32 Adam on
+ * minibatches. */
 typedef struct cfd_mlp {   /* row-major [out][in]; 1377 floats in this order */
   float w1[32 * 8];
   float b1[32];
