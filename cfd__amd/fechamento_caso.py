@@ -396,7 +396,10 @@ def avaliar(fech: Fechamento, seed: int = AVALIACAO_SEED, passos_grossos: int = 
     st_f = out["fino_fp32_smagorinsky"]["st_comprimento"]["lateral"]
     err = {k: (v["cd"] - cd_f) / cd_f for k, v in out.items() if k.startswith("grosso")}
     ganho = 16.0 * out["grosso_fp16sr_aprendido"]["mlups"] / out["fino_fp32_smagorinsky"]["mlups"]
-    aceito = abs(err["grosso_fp16sr_aprendido"]) <= abs(err["grosso_fp16sr_smagorinsky"])
+    # the judge compares ensemble means when there is an ensemble
+    err_c = {k: (float(np.mean(v["cd_conjunto"])) - cd_f) / cd_f
+             for k, v in out.items() if k.startswith("grosso")}
+    aceito = abs(err_c["grosso_fp16sr_aprendido"]) <= abs(err_c["grosso_fp16sr_smagorinsky"])
     out["resumo"] = {
         "cd_fino": cd_f,
         "erro_cd": err,
@@ -407,8 +410,7 @@ def avaliar(fech: Fechamento, seed: int = AVALIACAO_SEED, passos_grossos: int = 
         "ganho_vs_fino": ganho,
         "juiz": "aprendido" if aceito else "smagorinsky (o aprendido ficou mais longe do fino)",
         "erro_esteira": {k: v["erro_esteira"] for k, v in out.items() if k.startswith("grosso")},
-        "erro_cd_conjunto": {k: (float(np.mean(v["cd_conjunto"])) - cd_f) / cd_f
-                             for k, v in out.items() if k.startswith("grosso")},
+        "erro_cd_conjunto": err_c,
         "erro_padrao_conjunto": {
             k: (float(np.std(v["cd_conjunto"], ddof=1) / math.sqrt(len(v["cd_conjunto"])) / cd_f)
                 if len(v["cd_conjunto"]) > 1 else None)
