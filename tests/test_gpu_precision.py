@@ -183,3 +183,56 @@ def test_stochastic_rounding_is_reproducible_and_unbiased():
         s.destruir()
     assert abs(m[:, 0].mean() - 0.05) < 2e-6 and abs(m[:, 1:3].mean()) < 2e-6
     assert np.abs(m[:, 0] - 0.05).max() < 5e-5
+
+
+def test_stochastic_rounding_of_the_step_is_unbiased_per_value():
+    """E[stored | exact] = exact for the step's float16 stores: one step of a
+    random all-fluid state in fp16sr against the same step in float32 (which
+    differs from the float16 kernel's own float32 result by a few float32
+    ulp, 1e-4 of a float16 ulp).  Binned by where the exact value falls
+    between its two float16 neighbours, the mean rounding error of every bin
+    is zero to within its sampling error; round to nearest would give
+    -frac below the midpoint and 1 - frac above it."""
+    from cfd__amd.solver import SolverMI355X
+    g = {"nx": 64, "ny": 32, "nz": 32}
+    N = 64 * 32 * 32
+    rng = np.random.default_rng(8)
+    # a non-equilibrium state whose values are exact float16 multiples of 2^-12
+    g0 = (lattice.W[:, None] * (0.02 * rng.standard_normal((19, N)))).astype(np.float32)
+    g0 = ((g0 * 4096.0).astype(np.float16).astype(np.float32) / 4096.0).astype(np.float32)
+    out = {}
+    for prec in ("fp32", "fp16sr"):
+        s = SolverMI355X.criar(g, precisao=prec)
+        try:
+            s.definirTipos(np.zeros(N, np.uint32))
+            s.configurar(omegaPlus=1.3, lesCs=0.1, inletU=[0.0, 0.0, 0.0])
+            s.frente = "A"
+            s.definirPopulacoes(g0)
+            s.passo(1)
+            out[prec] = s.lerPopulacoes().astype(np.float64) * 4096.0
+        finally:
+            s.destruir()
+    v, got = out["fp32"].ravel(), out["fp16sr"].ravel()
+    lo = v.astype(np.float16)
+    lo = np.where(lo.astype(np.float64) > v, np.nextafter(lo, np.float16(-np.inf)), lo)
+    hi = np.nextafter(lo, np.float16(np.inf))
+    lo64, hi64 = lo.astype(np.float64), hi.astype(np.float64)
+    ulp = hi64 - lo64
+    frac = (v - lo64) / np.where(ulp > 0, ulp, 1.0)
+    # the float32 reference rounds its collision unlike the float16 kernel
+    # (subtract-first against gathered TRT terms): a few float32 ulp of the
+    # terms, which is a visible part of a float16 ulp only for results much
+    # smaller than their terms or next to a float16 value; those are left out
+    ok = (np.isfinite(ulp) & (ulp > 0) & (np.abs(v) >= 0.25) & (frac > 0.02) & (frac < 0.98))
+    v, got, lo64, hi64, ulp, frac = v[ok], got[ok], lo64[ok], hi64[ok], ulp[ok], frac[ok]
+    assert ok.mean() > 0.3, ok.mean()
+    other = ~((got == lo64) | (got == hi64))           # not one of the two neighbours
+    assert other.mean() < 1e-4, (int(other.sum()), v[other][:5], got[other][:5])
+    err = (got - v) / ulp
+    edges = np.linspace(0.0, 1.0, 11)
+    for a, b in zip(edges[:-1], edges[1:]):
+        sel = (frac >= a) & (frac < b)
+        n = int(sel.sum())
+        assert n > 10000
+        sigma = 0.5 / math.sqrt(n)
+        assert abs(err[sel].mean()) < 6 * sigma + 2e-3, (a, b, err[sel].mean(), sigma)
