@@ -13,11 +13,15 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libcfdmi.so"
 
 # CFD_ABI_VERSION of include/cfd_mi355x.h that the structs below follow
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 CFD_STORE_FP32 = 0
 CFD_STORE_FP16 = 1
 CFD_ROUND_STOCHASTIC = 2
+
+# cfd_mlp.activation
+CFD_ACT_SILU = 0
+CFD_ACT_HARD_SILU = 1
 CFD_ROUND_EXACT = 4
 
 
@@ -55,9 +59,11 @@ class BatchTuning(C.Structure):
 
 
 class Mlp(C.Structure):
-    """cfd_mlp: 1377 float32 in the order w1[32][8] b1 w2[32][32] b2 w3 b3."""
+    """cfd_mlp: 1377 float32 in the order w1[32][8] b1 w2[32][32] b2 w3 b3,
+    then the hidden activation (CFD_ACT_SILU or CFD_ACT_HARD_SILU)."""
     _fields_ = [("w1", C.c_float * 256), ("b1", C.c_float * 32), ("w2", C.c_float * 1024),
-                ("b2", C.c_float * 32), ("w3", C.c_float * 32), ("b3", C.c_float)]
+                ("b2", C.c_float * 32), ("w3", C.c_float * 32), ("b3", C.c_float),
+                ("activation", C.c_int32)]
 
 
 class Closure(C.Structure):

@@ -67,6 +67,7 @@ struct cfd_solver {
   double* host_res = nullptr; // pinned, 4 doubles
   // learned closure (off while mlp == nullptr)
   MlpPacked* mlp = nullptr;
+  int mlp_act = CFD_ACT_SILU;  // its hidden activation
   uint8_t* wall8 = nullptr;   // wall distance, 1/8 cell, saturating at 31.875
   bool wall_given = false;    // wall8 holds the caller's distances, not the types' transform
   unsigned long long* fallbacks = nullptr;
@@ -200,8 +201,9 @@ const char* cfd_last_error(void) { return g_err.c_str(); }
 // Lets the voxelizer translation unit report through cfd_last_error().
 void cfd__set_error(const char* msg) { g_err = msg ? msg : ""; }
 
-// 0.2: cfd_tuning gained rotate; 0.3: cfd_abi_version, default launch shape per set
-const char* cfd_version(void) { return "cfd_mi355x 0.3 gfx950"; }
+// 0.2: cfd_tuning gained rotate; 0.3: cfd_abi_version, default launch shape per set;
+// 0.4: cfd_mlp.activation
+const char* cfd_version(void) { return "cfd_mi355x 0.4 gfx950"; }
 
 int cfd_abi_version(void) { return CFD_ABI_VERSION; }
 
@@ -546,7 +548,8 @@ static int enqueue_steps(cfd_solver* s, int n) {
     s->params.sr_seed = ++s->sr_step * 0x9E3779B9u;   // a new rounding draw per step
     if (s->mlp)
       HIPCK(launch_step_closure(s->store, s->pop[f], s->pop[f ^ 1], s->tipo8, s->wall8, s->mlp,
-                                s->macros, false, s->geom, s->params, s->fallbacks, s->stream));
+                                s->mlp_act, s->macros, false, s->geom, s->params, s->fallbacks,
+                                s->stream));
     else
       HIPCK(launch_step(s->store, s->pop[f], s->pop[f ^ 1], s->tipo8, s->macros, false,
                         s->geom, s->params, s->tuning, s->tuning.sweep && (s->sr_step & 1),
@@ -703,12 +706,15 @@ int cfd_set_closure(cfd_solver* s, const cfd_closure* c) {
         !(c->in_lo[k] <= c->in_hi[k]))
       return fail(CFD_E_ARG, "cfd_set_closure: feature %d has a bad normalization or range", k);
   if (!(c->out_scale >= 0.0f)) return fail(CFD_E_ARG, "cfd_set_closure: out_scale must be >= 0");
+  if (!known_activation(c->mlp.activation))
+    return fail(CFD_E_ARG, "cfd_set_closure: unknown activation %d", (int)c->mlp.activation);
   int rc = ensure_closure_buffers(s);
   if (rc) return rc;
   MlpPacked pk;
   pack_closure(*c, &pk);
   if (!s->mlp) HIPCK(hipMalloc(&s->mlp, sizeof(MlpPacked)));
   HIPCK(hipMemcpy(s->mlp, &pk, sizeof pk, hipMemcpyHostToDevice));
+  s->mlp_act = c->mlp.activation;
   return CFD_OK;
 }
 

@@ -12,10 +12,13 @@
 // the host).  Layer 3 puts w3 in every row of A, so every lane group receives
 // each cell's output and the owning lane keeps its own: no shuffle.
 //
-// The network runs in base 2 (pack_closure): layer 1 and 2 produce
+// The hidden activation is the network's own (cfd_mlp.activation).  A SiLU
+// network runs in base 2 (pack_closure): layer 1 and 2 produce
 // u = -log2(e) z, whose SiLU is u / (1 + 2^u) = -log2(e) SiLU(z), and w3
 // carries the -ln 2 back, so a SiLU is one exponential, one add, one
-// reciprocal and one multiply.
+// reciprocal and one multiply.  A hard-SiLU network, z * clamp(z/6 + 1/2,
+// 0, 1), runs on z itself: one fma whose result the hardware clamps to
+// [0, 1] and one multiply.
 //
 // The closure step (k_step_closure) is the step of lbm_kernels.hip with the
 // eddy viscosity taken from the MLP: features are the non-equilibrium stress
@@ -50,10 +53,18 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------ inference
 
-// SiLU in base 2 on u = -log2(e) z: u / (1 + 2^u) = -log2(e) SiLU(z).  For
-// u beyond the exponential's range 2^u is +inf and the result 0.
-__device__ __forceinline__ float silu2_fast(float u) {
-  return u * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(u));
+constexpr float kSixth = 1.0f / 6.0f, kThird = 1.0f / 3.0f;
+
+// The hidden activation of the matrix-core path.  SiLU in base 2 on
+// u = -log2(e) z: u / (1 + 2^u) = -log2(e) SiLU(z); for u beyond the
+// exponential's range 2^u is +inf and the result 0.  Hard-SiLU on z:
+// z * clamp(z/6 + 1/2, 0, 1) (a NaN z gives NaN).
+template <int ACT>
+__device__ __forceinline__ float act_fast(float u) {
+  if constexpr (ACT == CFD_ACT_HARD_SILU)
+    return u * __builtin_amdgcn_fmed3f(fmaf(u, kSixth, 0.5f), 0.0f, 1.0f);
+  else
+    return u * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(u));
 }
 
 __device__ __forceinline__ float softplus_fast(float z) {
@@ -68,18 +79,20 @@ __device__ __forceinline__ v8bf ld8(const uint16_t* p) {
 
 __device__ __forceinline__ v4f ld4(const float* p) { return *reinterpret_cast<const v4f*>(p); }
 
-__device__ __forceinline__ v8bf pack_silu(const v4f& a, const v4f& b) {
+template <int ACT>
+__device__ __forceinline__ v8bf pack_act(const v4f& a, const v4f& b) {
   v8bf r;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    r[v] = (__bf16)silu2_fast(a[v]);
-    r[v + 4] = (__bf16)silu2_fast(b[v]);
+    r[v] = (__bf16)act_fast<ACT>(a[v]);
+    r[v + 4] = (__bf16)act_fast<ACT>(b[v]);
   }
   return r;
 }
 
 // z (before softplus) of this lane's cell; xb = its normalized features in
 // bf16 (zero for lanes that do not use the result).  All 64 lanes must call.
+template <int ACT>
 __device__ __forceinline__ float mlp_wave(const v8bf& xb, const MlpPacked& W) {
   const int l = threadIdx.x & 63, grp = l >> 4, row = l & 15;
   const v4f zero4 = {0.f, 0.f, 0.f, 0.f};
@@ -96,11 +109,11 @@ __device__ __forceinline__ float mlp_wave(const v8bf& xb, const MlpPacked& W) {
     const int r1 = grp == gs ? row : 16;
     const v4f h0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(W.w1[0][r1]), xb, b1a, 0, 0, 0);
     const v4f h1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(W.w1[1][r1]), xb, b1b, 0, 0, 0);
-    const v8bf x2 = pack_silu(h0, h1);
+    const v8bf x2 = pack_act<ACT>(h0, h1);
     // layer 2
     const v4f g0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2a, x2, b2a, 0, 0, 0);
     const v4f g1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2b, x2, b2b, 0, 0, 0);
-    const v8bf x3 = pack_silu(g0, g1);
+    const v8bf x3 = pack_act<ACT>(g0, g1);
     // layer 3: every row of A is w3, so every row of the result holds the
     // outputs of the 16 cells; the lanes of group gs own them
     const v4f zz = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, x3, zero4, 0, 0, 0);
@@ -121,7 +134,7 @@ __device__ __forceinline__ void load_mlp(MlpPacked& dst, const MlpPacked* __rest
 
 constexpr int kClosureBlock = 256;
 
-template <class S, bool MACROS, int LB>
+template <class S, bool MACROS, int LB, int ACT>
 __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
                                                   const GLOBAL typename S::T* __restrict__ src,
                                                   GLOBAL typename S::T* __restrict__ dst,
@@ -197,7 +210,7 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
 #pragma unroll
   for (int i = 0; i < Q; ++i) s_gi[i][threadIdx.x] = gi[i];
   asm volatile("" ::: "memory");
-  const float nu = W.out_scale * softplus_fast(mlp_wave(xb, W));
+  const float nu = W.out_scale * softplus_fast(mlp_wave<ACT>(xb, W));
   asm volatile("" ::: "memory");
 #pragma unroll
   for (int i = 0; i < Q; ++i) gi[i] = s_gi[i][threadIdx.x];
@@ -247,7 +260,7 @@ __device__ __forceinline__ void step_cell_closure(uint32_t cell_in, bool live,
   }
 }
 
-template <class S, bool MACROS, int LB>
+template <class S, bool MACROS, int LB, int ACT>
 __global__ __launch_bounds__(kClosureBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_step_closure(
     const typename S::T* __restrict__ src, typename S::T* __restrict__ dst,
     const uint8_t* __restrict__ tipo, const uint8_t* __restrict__ wall8,
@@ -256,29 +269,44 @@ __global__ __launch_bounds__(kClosureBlock) __attribute__((amdgpu_waves_per_eu(5
   __shared__ MlpPacked W;
   load_mlp(W, mlp);
   const uint32_t cell = blockIdx.x * kClosureBlock + threadIdx.x;
-  step_cell_closure<S, MACROS, LB>(cell, cell < g.N, gptr(src), gptr(dst), gptr(tipo),
-                                   gptr(wall8), macros, g, p, W, fallbacks);
+  step_cell_closure<S, MACROS, LB, ACT>(cell, cell < g.N, gptr(src), gptr(dst), gptr(tipo),
+                                        gptr(wall8), macros, g, p, W, fallbacks);
 }
 
-hipError_t launch_step_closure(Store sto, const void* src, void* dst, const uint8_t* tipo8,
+template <int ACT>
+static void launch_closure_act(Store sto, const void* src, void* dst, const uint8_t* tipo8,
                                const uint8_t* wall8, const MlpPacked* mlp, float4* macros,
                                bool write_macros, const Geom& g, const StepParams& p,
                                unsigned long long* fallbacks, hipStream_t st) {
   const dim3 grid((g.N + kClosureBlock - 1) / kClosureBlock);
   CFD_DISPATCH(sto, g, {
     if (write_macros)
-      hipLaunchKernelGGL((k_step_closure<S, true, L>), grid, dim3(kClosureBlock), 0, st,
+      hipLaunchKernelGGL((k_step_closure<S, true, L, ACT>), grid, dim3(kClosureBlock), 0, st,
                          static_cast<const typename S::T*>(src), static_cast<typename S::T*>(dst),
                          tipo8, wall8, mlp, macros, g, p, fallbacks);
     else
-      hipLaunchKernelGGL((k_step_closure<S, false, L>), grid, dim3(kClosureBlock), 0, st,
+      hipLaunchKernelGGL((k_step_closure<S, false, L, ACT>), grid, dim3(kClosureBlock), 0, st,
                          static_cast<const typename S::T*>(src), static_cast<typename S::T*>(dst),
                          tipo8, wall8, mlp, macros, g, p, fallbacks);
   });
+}
+
+hipError_t launch_step_closure(Store sto, const void* src, void* dst, const uint8_t* tipo8,
+                               const uint8_t* wall8, const MlpPacked* mlp, int act,
+                               float4* macros, bool write_macros, const Geom& g,
+                               const StepParams& p, unsigned long long* fallbacks,
+                               hipStream_t st) {
+  if (act == CFD_ACT_HARD_SILU)
+    launch_closure_act<CFD_ACT_HARD_SILU>(sto, src, dst, tipo8, wall8, mlp, macros, write_macros,
+                                          g, p, fallbacks, st);
+  else
+    launch_closure_act<CFD_ACT_SILU>(sto, src, dst, tipo8, wall8, mlp, macros, write_macros, g,
+                                     p, fallbacks, st);
   return hipGetLastError();
 }
 
 // Inference alone on given raw features (the same device path as the step).
+template <int ACT>
 __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ X, uint32_t n,
                                                   const MlpPacked* __restrict__ mlp,
                                                   float* __restrict__ nu,
@@ -296,7 +324,7 @@ __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ X, u
     inside = inside && xn >= W.lo[k] && xn <= W.hi[k];
     xb[k] = (__bf16)xn;
   }
-  const float out = W.out_scale * softplus_fast(mlp_wave(xb, W));
+  const float out = W.out_scale * softplus_fast(mlp_wave<ACT>(xb, W));
   if (live) {
     nu[s] = out;
     inside_out[s] = inside ? 1 : 0;
@@ -313,6 +341,18 @@ constexpr int kGradSlots = kMlpParams + 1;  // + the loss
 
 __device__ __forceinline__ float sigmoidf_exact(float a) { return 1.0f / (1.0f + expf(-a)); }
 
+// The hidden activation in float32 and its derivative (training).
+__device__ __forceinline__ float act_exact(float a, int act) {
+  if (act == CFD_ACT_HARD_SILU) return a * fminf(fmaxf(fmaf(a, kSixth, 0.5f), 0.0f), 1.0f);
+  return a * sigmoidf_exact(a);
+}
+
+__device__ __forceinline__ float act_grad(float a, int act) {
+  if (act == CFD_ACT_HARD_SILU) return a <= -3.0f ? 0.0f : (a >= 3.0f ? 1.0f : fmaf(a, kThird, 0.5f));
+  const float sg = sigmoidf_exact(a);
+  return sg * (1.0f + a * (1.0f - sg));
+}
+
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -327,7 +367,8 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ param
                                                  const float* __restrict__ X,
                                                  const float* __restrict__ Y, uint64_t n,
                                                  int random, uint64_t seed, uint32_t step,
-                                                 float inv_count, float* __restrict__ partial) {
+                                                 float inv_count, int act,
+                                                 float* __restrict__ partial) {
   __shared__ float P[kMlpParams];
   __shared__ float xs[64][kMlpIn];
   __shared__ float h1s[64][kMlpHidden], d1s[64][kMlpHidden];
@@ -360,7 +401,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ param
 #pragma unroll
     for (int k = 0; k < kMlpIn; ++k) acc += P[kOffW1 + o * kMlpIn + k] * x[k];
     a1[o] = acc;
-    h1[o] = acc * sigmoidf_exact(acc);
+    h1[o] = act_exact(acc, act);
   }
 #pragma unroll
   for (int o = 0; o < kMlpHidden; ++o) {
@@ -368,7 +409,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ param
 #pragma unroll
     for (int k = 0; k < kMlpHidden; ++k) acc += P[kOffW2 + o * kMlpHidden + k] * h1[k];
     a2[o] = acc;
-    h2[o] = acc * sigmoidf_exact(acc);
+    h2[o] = act_exact(acc, act);
   }
   float z = P[kOffB3];
 #pragma unroll
@@ -380,8 +421,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ param
   const float dz = live ? 2.0f * err * inv_count * sigmoidf_exact(z) : 0.0f;
 #pragma unroll
   for (int o = 0; o < kMlpHidden; ++o) {
-    const float sg = sigmoidf_exact(a2[o]);
-    d2s[t][o] = dz * P[kOffW3 + o] * (sg * (1.0f + a2[o] * (1.0f - sg)));
+    d2s[t][o] = dz * P[kOffW3 + o] * act_grad(a2[o], act);
     h2s[t][o] = h2[o];
   }
   __syncthreads();
@@ -390,8 +430,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ param
     float acc = 0.0f;
 #pragma unroll
     for (int o = 0; o < kMlpHidden; ++o) acc += P[kOffW2 + o * kMlpHidden + k] * d2s[t][o];
-    const float sg = sigmoidf_exact(a1[k]);
-    d1s[t][k] = acc * (sg * (1.0f + a1[k] * (1.0f - sg)));
+    d1s[t][k] = acc * act_grad(a1[k], act);
     h1s[t][k] = h1[k];
   }
 #pragma unroll
@@ -607,14 +646,16 @@ int efail(int code, const char* what, hipError_t e) {
 
 namespace cfd {
 
-// The base-2 form of the network (float32 products, then bf16 for the
-// matrix operands): W1, b1, b2 times -log2(e), so layers 1 and 2 produce
+// The matrix operands (float32 products, then bf16).  A SiLU network in its
+// base-2 form: W1, b1, b2 times -log2(e), so layers 1 and 2 produce
 // u = -log2(e) z; W2 unchanged (its input is -log2(e) SiLU and its output is
-// scaled like its bias); w3 times -ln 2.  Row 16 of each W1 tile stays zero.
+// scaled like its bias); w3 times -ln 2.  A hard-SiLU network as it is.  Row
+// 16 of each W1 tile stays zero.
 void pack_closure(const cfd_closure& c, MlpPacked* out) {
   std::memset(out, 0, sizeof *out);
   const cfd_mlp& m = c.mlp;
-  const float nl2e = -kLog2e, nln2 = -kLn2;
+  const bool base2 = m.activation == CFD_ACT_SILU;
+  const float nl2e = base2 ? -kLog2e : 1.0f, nln2 = base2 ? -kLn2 : 1.0f;
   for (int t = 0; t < 2; ++t)
     for (int i = 0; i < 16; ++i) {
       for (int v = 0; v < 8; ++v)
@@ -636,13 +677,18 @@ void pack_closure(const cfd_closure& c, MlpPacked* out) {
   }
 }
 
+bool known_activation(int32_t act) {
+  return act == CFD_ACT_SILU || act == CFD_ACT_HARD_SILU;
+}
+
 }  // namespace cfd
 
 extern "C" {
 
 int cfd_mlp_infer(const float* X, size_t n, const cfd_closure* c, float* nu, uint8_t* inside) {
   cfd__set_error("");
-  if (!X || !c || !nu || !inside || n == 0 || n >= (1ull << 31)) {
+  if (!X || !c || !nu || !inside || n == 0 || n >= (1ull << 31) ||
+      !known_activation(c->mlp.activation)) {
     cfd__set_error("cfd_mlp_infer: bad arguments");
     return CFD_E_ARG;
   }
@@ -658,8 +704,13 @@ int cfd_mlp_infer(const float* X, size_t n, const cfd_closure* c, float* nu, uin
   if (e == hipSuccess) e = hipMemcpy(dX, X, n * kMlpIn * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dm, &pk, sizeof pk, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, 0, dX,
-                       (uint32_t)n, dm, dnu, din);
+    const dim3 grid((unsigned)((n + 63) / 64));
+    if (c->mlp.activation == CFD_ACT_HARD_SILU)
+      hipLaunchKernelGGL(k_mlp_infer<CFD_ACT_HARD_SILU>, grid, dim3(64), 0, 0, dX, (uint32_t)n,
+                         dm, dnu, din);
+    else
+      hipLaunchKernelGGL(k_mlp_infer<CFD_ACT_SILU>, grid, dim3(64), 0, 0, dX, (uint32_t)n, dm,
+                         dnu, din);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(nu, dnu, n * sizeof(float), hipMemcpyDeviceToHost);
@@ -682,6 +733,7 @@ static int mlp_run(const float* X, const float* Y, size_t n, const cfd_train_cfg
   float *dX = nullptr, *dY = nullptr, *dP = nullptr, *dm = nullptr, *dv = nullptr,
         *dpart = nullptr, *dgrad = nullptr, *dloss = nullptr;
   const int steps = train ? cfg->steps : 1;
+  const int act = params->activation;
   hipError_t e = hipMalloc(&dX, n * kMlpIn * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&dY, n * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&dP, kMlpParams * sizeof(float));
@@ -700,13 +752,13 @@ static int mlp_run(const float* X, const float* Y, size_t n, const cfd_train_cfg
   for (int s = 1; e == hipSuccess && s <= steps; ++s) {
     if (train) {
       hipLaunchKernelGGL(k_mlp_grad, dim3(nwg), dim3(64), 0, 0, dP, dX, dY, (uint64_t)n, 1,
-                         (uint64_t)cfg->seed, (uint32_t)s, 1.0f / (float)batch, dpart);
+                         (uint64_t)cfg->seed, (uint32_t)s, 1.0f / (float)batch, act, dpart);
       hipLaunchKernelGGL(k_mlp_adam, dim3(agrid), dim3(256), 0, 0, dP, dm, dv, dpart, nwg,
                          (uint32_t)s, cfg->lr, cfg->beta1, cfg->beta2, cfg->eps,
                          1.0f / (float)batch, (float*)nullptr, dloss);
     } else {
       hipLaunchKernelGGL(k_mlp_grad, dim3(nwg), dim3(64), 0, 0, dP, dX, dY, (uint64_t)n, 0,
-                         (uint64_t)0, 1u, 1.0f / (float)n, dpart);
+                         (uint64_t)0, 1u, 1.0f / (float)n, act, dpart);
       hipLaunchKernelGGL(k_mlp_adam, dim3(agrid), dim3(256), 0, 0, dP, dm, dv, dpart, nwg, 1u,
                          0.0f, 0.9f, 0.999f, 1e-8f, 1.0f / (float)n, dgrad, dloss);
     }
@@ -717,8 +769,10 @@ static int mlp_run(const float* X, const float* Y, size_t n, const cfd_train_cfg
     e = hipMemcpy(params, dP, kMlpParams * sizeof(float), hipMemcpyDeviceToHost);
   if (e == hipSuccess && train && loss_curve)
     e = hipMemcpy(loss_curve, dloss, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost);
-  if (e == hipSuccess && !train && grad)
+  if (e == hipSuccess && !train && grad) {
     e = hipMemcpy(grad, dgrad, kMlpParams * sizeof(float), hipMemcpyDeviceToHost);
+    grad->activation = act;
+  }
   if (e == hipSuccess && !train && loss_full) {
     float l = 0.f;
     e = hipMemcpy(&l, dloss, sizeof(float), hipMemcpyDeviceToHost);
@@ -734,8 +788,9 @@ int cfd_mlp_train(const float* X, const float* y, size_t n, const cfd_train_cfg*
                   cfd_mlp* params, float* loss_curve) {
   cfd__set_error("");
   if (!X || !y || !cfg || !params || n == 0 || cfg->steps < 1 || cfg->batch < 64 ||
-      cfg->batch % 64 != 0 || !(cfg->lr > 0.0f)) {
-    cfd__set_error("cfd_mlp_train: bad arguments (batch a multiple of 64, steps >= 1, lr > 0)");
+      cfg->batch % 64 != 0 || !(cfg->lr > 0.0f) || !known_activation(params->activation)) {
+    cfd__set_error("cfd_mlp_train: bad arguments (batch a multiple of 64, steps >= 1, lr > 0, "
+                   "a known activation)");
     return CFD_E_ARG;
   }
   return mlp_run(X, y, n, cfg, params, nullptr, nullptr, loss_curve);
@@ -744,7 +799,8 @@ int cfd_mlp_train(const float* X, const float* y, size_t n, const cfd_train_cfg*
 int cfd_mlp_grad(const float* X, const float* y, size_t n, const cfd_mlp* params, cfd_mlp* grad,
                  double* loss) {
   cfd__set_error("");
-  if (!X || !y || !params || !grad || n == 0 || n >= (1ull << 31)) {
+  if (!X || !y || !params || !grad || n == 0 || n >= (1ull << 31) ||
+      !known_activation(params->activation)) {
     cfd__set_error("cfd_mlp_grad: bad arguments");
     return CFD_E_ARG;
   }

@@ -179,10 +179,12 @@ struct MlpPacked {
   float shift[8], scale[8], lo[8], hi[8];
 };
 void pack_closure(const ::cfd_closure& c, MlpPacked* out);
+bool known_activation(int32_t act);  // CFD_ACT_SILU or CFD_ACT_HARD_SILU
 hipError_t launch_step_closure(Store sto, const void* src, void* dst, const uint8_t* tipo8,
-                               const uint8_t* wall8, const MlpPacked* mlp, float4* macros,
-                               bool write_macros, const Geom& g, const StepParams& p,
-                               unsigned long long* fallbacks, hipStream_t st);
+                               const uint8_t* wall8, const MlpPacked* mlp, int act,
+                               float4* macros, bool write_macros, const Geom& g,
+                               const StepParams& p, unsigned long long* fallbacks,
+                               hipStream_t st);
 // Wall distance of the closure from the type field: gx (N bytes) and h2
 // (N uint16) are scratch, table[0..1024] maps d^2 (capped at 1024) to wall8.
 hipError_t launch_wall_distance(const uint8_t* tipo8, uint8_t* wall8, uint8_t* gx, uint16_t* h2,

@@ -4,7 +4,10 @@ The closure replaces the Smagorinsky eddy viscosity of the step
 (ir.js:262-301) by nu_t = out_scale * softplus(MLP(x_n)), with x_n the
 normalized features (Pi_xx, Pi_yy, Pi_zz, Pi_xy, Pi_xz, Pi_yz) / rho of the
 non-equilibrium stress, |S| = 3 Q_s / (2 rho tau_s) and the wall distance in
-cells.  Training (cfd_mlp_train) and inference (inside the step, bf16 on the
+cells.  The hidden activation is SiLU or hard-SiLU, z * clamp(z/6 + 1/2, 0, 1)
+(``ativacao``); hard-SiLU costs the step's matrix-core path one clamped fma
+and one multiply per hidden unit instead of an exponential and a reciprocal.
+Training (cfd_mlp_train) and inference (inside the step, bf16 on the
 matrix cores) run on the GPU through the C ABI; this module prepares data and
 parameters.
 
@@ -23,11 +26,18 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import Closure, Mlp, TrainCfg, check, lib, ptr
+from ._lib import CFD_ACT_HARD_SILU, CFD_ACT_SILU, Closure, Mlp, TrainCfg, check, lib, ptr
 
 N_PARAMS = 1377
 FEATURES = ("Pi_xx", "Pi_yy", "Pi_zz", "Pi_xy", "Pi_xz", "Pi_yz", "|S|", "wall_distance")
 OFF = {"w1": 0, "b1": 256, "w2": 288, "b2": 1312, "w3": 1344, "b3": 1376}
+ATIVACOES = {"silu": CFD_ACT_SILU, "hard_silu": CFD_ACT_HARD_SILU}
+
+
+def _codigo(ativacao: str) -> int:
+    if ativacao not in ATIVACOES:
+        raise ValueError(f"unknown activation {ativacao!r} (one of {sorted(ATIVACOES)})")
+    return ATIVACOES[ativacao]
 
 
 def iniciar_mlp(seed: int = 0, bias_saida: float = -2.0) -> np.ndarray:
@@ -42,10 +52,11 @@ def iniciar_mlp(seed: int = 0, bias_saida: float = -2.0) -> np.ndarray:
     return p
 
 
-def _mlp_struct(p: np.ndarray) -> Mlp:
+def _mlp_struct(p: np.ndarray, ativacao: str = "silu") -> Mlp:
     m = Mlp()
     C.memmove(C.addressof(m), np.ascontiguousarray(p, dtype=np.float32).ctypes.data,
               N_PARAMS * 4)
+    m.activation = _codigo(ativacao)
     return m
 
 
@@ -55,26 +66,28 @@ def _mlp_array(m: Mlp) -> np.ndarray:
     return out
 
 
-def gradiente(Xn: np.ndarray, y: np.ndarray, params: np.ndarray) -> tuple[np.ndarray, float]:
+def gradiente(Xn: np.ndarray, y: np.ndarray, params: np.ndarray,
+              ativacao: str = "silu") -> tuple[np.ndarray, float]:
     """Full-batch gradient of mean (softplus(MLP(Xn)) - y)^2 (float32 kernels)."""
     Xn = np.ascontiguousarray(Xn, dtype=np.float32)
     y = np.ascontiguousarray(y, dtype=np.float32)
     g = Mlp()
     loss = C.c_double()
-    check(lib.cfd_mlp_grad(ptr(Xn), ptr(y), y.size, C.byref(_mlp_struct(params)), C.byref(g),
-                           C.byref(loss)))
+    check(lib.cfd_mlp_grad(ptr(Xn), ptr(y), y.size, C.byref(_mlp_struct(params, ativacao)),
+                           C.byref(g), C.byref(loss)))
     return _mlp_array(g), loss.value
 
 
 def treinar_mlp(Xn: np.ndarray, y: np.ndarray, params: np.ndarray, passos: int = 20000,
                 lote: int = 4096, lr: float = 1e-3, seed: int = 1,
-                beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
+                beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8,
+                ativacao: str = "silu"):
     """Adam on minibatches drawn on the device; returns (params, loss curve)."""
     Xn = np.ascontiguousarray(Xn, dtype=np.float32)
     y = np.ascontiguousarray(y, dtype=np.float32)
     cfg = TrainCfg(int(passos), int(lote), float(lr), float(beta1), float(beta2), float(eps),
                    int(seed))
-    m = _mlp_struct(params)
+    m = _mlp_struct(params, ativacao)
     curva = np.empty(int(passos), dtype=np.float32)
     check(lib.cfd_mlp_train(ptr(Xn), ptr(y), y.size, C.byref(cfg), C.byref(m), ptr(curva)))
     return _mlp_array(m), curva
@@ -88,10 +101,12 @@ class Fechamento:
     lo: np.ndarray
     hi: np.ndarray
     out_scale: float
+    ativacao: str = "silu"
     historico: dict = field(default_factory=dict)
 
     @classmethod
-    def novo(cls, X: np.ndarray, y_nu: np.ndarray, seed: int = 0, margem: float = 0.5):
+    def novo(cls, X: np.ndarray, y_nu: np.ndarray, seed: int = 0, margem: float = 0.5,
+             ativacao: str = "silu"):
         """Normalization (mean / std per feature), the trained range (the
         data's min..max of x_n widened by `margem` standard deviations) and
         the output scale (std of the target) from the training data."""
@@ -103,8 +118,9 @@ class Fechamento:
         lo = xn.min(axis=0) - margem
         hi = xn.max(axis=0) + margem
         out = float(np.std(y_nu)) or 1.0
+        _codigo(ativacao)
         return cls(iniciar_mlp(seed), mu.astype(np.float32), (1 / sd).astype(np.float32),
-                   lo.astype(np.float32), hi.astype(np.float32), out)
+                   lo.astype(np.float32), hi.astype(np.float32), out, ativacao)
 
     def normalizar(self, X: np.ndarray) -> np.ndarray:
         return ((np.asarray(X, np.float32) - self.shift) * self.scale).astype(np.float32)
@@ -112,13 +128,14 @@ class Fechamento:
     def treinar(self, X: np.ndarray, y_nu: np.ndarray, passos: int = 20000, lote: int = 4096,
                 lr: float = 1e-3, seed: int = 1) -> np.ndarray:
         self.params, curva = treinar_mlp(self.normalizar(X), np.asarray(y_nu) / self.out_scale,
-                                         self.params, passos, lote, lr, seed)
+                                         self.params, passos, lote, lr, seed,
+                                         ativacao=self.ativacao)
         self.historico["perda"] = curva
         return curva
 
     def c_struct(self) -> Closure:
         c = Closure()
-        c.mlp = _mlp_struct(self.params)
+        c.mlp = _mlp_struct(self.params, self.ativacao)
         for k in range(8):
             c.in_shift[k] = float(self.shift[k])
             c.in_scale[k] = float(self.scale[k])
@@ -139,7 +156,8 @@ class Fechamento:
 
     def salvar(self, caminho) -> None:
         np.savez(caminho, params=self.params, shift=self.shift, scale=self.scale, lo=self.lo,
-                 hi=self.hi, out_scale=np.float32(self.out_scale))
+                 hi=self.hi, out_scale=np.float32(self.out_scale),
+                 ativacao=np.array(self.ativacao))
 
     def salvarJson(self, caminho) -> None:
         """The model as JSON for the Node class (carregarFechamento in
@@ -154,14 +172,17 @@ class Fechamento:
             return [float(str(v)) for v in a]
         d = {"params": f32(self.params), "shift": f32(self.shift), "scale": f32(self.scale),
              "lo": f32(self.lo), "hi": f32(self.hi),
-             "out_scale": f32([self.out_scale])[0]}
+             "out_scale": f32([self.out_scale])[0], "ativacao": self.ativacao}
         with open(caminho, "w") as f:
             json.dump(d, f)
 
     @classmethod
     def carregar(cls, caminho) -> "Fechamento":
         d = np.load(caminho, allow_pickle=False)
-        return cls(d["params"], d["shift"], d["scale"], d["lo"], d["hi"], float(d["out_scale"]))
+        ativacao = str(d["ativacao"]) if "ativacao" in d.files else "silu"
+        _codigo(ativacao)
+        return cls(d["params"], d["shift"], d["scale"], d["lo"], d["hi"], float(d["out_scale"]),
+                   ativacao)
 
 
 def distancia_parede(tipo: np.ndarray, grade: dict, paredes=(1, 4, 7)) -> np.ndarray:

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <string>
 
@@ -429,9 +430,9 @@ napi_value GetTuning(napi_env env, napi_callback_info info) {
   return out;
 }
 
-// setClosure(h, Float32Array(1410) | null): the cfd_closure struct flat, in
-// its field order (the 1377 MLP parameters, in_shift, in_scale, in_lo,
-// in_hi, out_scale); null returns to Smagorinsky.
+// setClosure(h, Float32Array(1411) | null): the cfd_closure fields flat, in
+// their order (the 1377 MLP parameters, the activation code as a number,
+// in_shift, in_scale, in_lo, in_hi, out_scale); null returns to Smagorinsky.
 napi_value SetClosure(napi_env env, napi_callback_info info) {
   napi_value a[2];
   if (!args(env, info, 2, a)) return nullptr;
@@ -446,13 +447,25 @@ napi_value SetClosure(napi_env env, napi_callback_info info) {
   float* d = nullptr;
   size_t n = 0;
   if (!view(env, a[1], napi_float32_array, &d, &n)) return nullptr;
-  static_assert(sizeof(cfd_closure) == 1410 * sizeof(float), "cfd_closure is 1410 floats");
-  if (n != 1410) {
-    napi_throw_range_error(env, nullptr, "cfdmi addon: a closure is 1410 float32 values");
+  constexpr size_t kParams = offsetof(cfd_mlp, activation) / sizeof(float);
+  static_assert(kParams == 1377, "cfd_mlp: 1377 parameters before the activation");
+  if (n != kParams + 1 + 33) {
+    napi_throw_range_error(env, nullptr, "cfdmi addon: a closure is 1411 float32 values");
+    return nullptr;
+  }
+  if (d[kParams] != 0.0f && d[kParams] != 1.0f) {
+    napi_throw_range_error(env, nullptr, "cfdmi addon: the activation code is 0 or 1");
     return nullptr;
   }
   cfd_closure c;
-  std::memcpy(&c, d, sizeof c);
+  std::memcpy(&c.mlp, d, kParams * sizeof(float));
+  c.mlp.activation = (int32_t)d[kParams];
+  const float* r = d + kParams + 1;
+  std::memcpy(c.in_shift, r, sizeof c.in_shift);
+  std::memcpy(c.in_scale, r + 8, sizeof c.in_scale);
+  std::memcpy(c.in_lo, r + 16, sizeof c.in_lo);
+  std::memcpy(c.in_hi, r + 24, sizeof c.in_hi);
+  c.out_scale = r[32];
   CFD_OK_OR_THROW(cfd_set_closure(h->s, &c));
   return nullptr;
 }

@@ -201,7 +201,8 @@ class SolverMI355X {
   }
 
   /** Learned sub-grid closure in every following step (no reference
-   *  counterpart): modelo = {params (1377), shift, scale, lo, hi (8 each),
+   *  counterpart): modelo = {params (1377), ativacao ('silu' or
+   *  'hard_silu', default 'silu'), shift, scale, lo, hi (8 each),
    *  outScale}, as carregarFechamento returns it; null returns to
    *  Smagorinsky.  Needs the wall distance (calcularDistanciaParede). */
   definirFechamento(modelo) {
@@ -209,7 +210,9 @@ class SolverMI355X {
       addon.setClosure(this._h, null);
       return;
     }
-    const c = new Float32Array(1410);
+    const codigo = ATIVACOES[modelo.ativacao === undefined ? 'silu' : modelo.ativacao];
+    if (codigo === undefined) throw new Error(`fechamento: ativacao desconhecida ${modelo.ativacao}`);
+    const c = new Float32Array(1411);
     const partes = [['params', 1377], ['shift', 8], ['scale', 8], ['lo', 8], ['hi', 8]];
     let k = 0;
     partes.forEach(([nome, n]) => {
@@ -217,6 +220,7 @@ class SolverMI355X {
       if (!v || v.length !== n) throw new Error(`fechamento: ${nome} deve ter ${n} valores`);
       c.set(v, k);
       k += n;
+      if (nome === 'params') c[k++] = codigo;
     });
     c[k] = modelo.outScale;
     addon.setClosure(this._h, c);
@@ -265,6 +269,8 @@ function voxelizar(positions, indices, g, opts) {
 }
 
 const VOXEL = { FLUIDO: 0, CASCA: 1, DENTRO: 2 };
+// hidden activation codes of cfd_mlp.activation (CFD_ACT_SILU, CFD_ACT_HARD_SILU)
+const ATIVACOES = { silu: 0, hard_silu: 1 };
 
 /** A closure model exported by cfd__amd.fechamento.Fechamento.salvarJson. */
 function carregarFechamento(caminho) {
@@ -272,7 +278,7 @@ function carregarFechamento(caminho) {
   return {
     params: Float32Array.from(d.params), shift: Float32Array.from(d.shift),
     scale: Float32Array.from(d.scale), lo: Float32Array.from(d.lo), hi: Float32Array.from(d.hi),
-    outScale: d.out_scale,
+    outScale: d.out_scale, ativacao: d.ativacao === undefined ? 'silu' : d.ativacao,
   };
 }
 

@@ -191,8 +191,9 @@ void cfd_destroy(cfd_solver *s);
 
 /* ---- learned sub-grid closure ----
  * No reference counterpart: the reference's only closure is the closed-form
- * Smagorinsky of ir.js:262-301.  A per-cell MLP 8 -> 32 -> 32 -> 1 (SiLU,
- * SiLU, softplus) gives the eddy viscosity nu_t; the step then relaxes with
+ * Smagorinsky of ir.js:262-301.  A per-cell MLP 8 -> 32 -> 32 -> 1 (two
+ * hidden activations, then softplus) gives the eddy viscosity nu_t; the step
+ * then relaxes with
  * tau = 1/omega_plus + 3 nu_t instead of the Smagorinsky tau.  Features of a
  * fluid cell: Pi_xx, Pi_yy, Pi_zz, Pi_xy, Pi_xz, Pi_yz of the non-equilibrium
  * stress divided by rho, the strain rate |S| = 3 Q_s / (2 rho tau_s) implied
@@ -200,10 +201,18 @@ void cfd_destroy(cfd_solver *s);
  * whose normalized features leave [in_lo, in_hi], or whose output is not
  * finite, keeps the Smagorinsky tau (counted by cfd_closure_fallbacks).
  * Inference uses bf16 operands with float32 accumulation on the matrix
- * cores, the network evaluated in base 2 (w1, b1, b2 times -log2(e) and w3
- * times -ln 2 in float32, then bf16: the same function, with the bf16
- * roundings of that form); training (cfd_mlp_train) is float32 Adam on
- * minibatches. */
+ * cores; training (cfd_mlp_train) is float32 Adam on minibatches.  The hidden
+ * activation is the network's own (cfd_mlp.activation):
+ *   CFD_ACT_SILU       z / (1 + e^-z), evaluated in base 2 on the matrix-core
+ *                      path (w1, b1, b2 times -log2(e) and w3 times -ln 2 in
+ *                      float32, then bf16: the same function, with the bf16
+ *                      roundings of that form);
+ *   CFD_ACT_HARD_SILU  z * clamp(z / 6 + 1/2, 0, 1), the piecewise-quadratic
+ *                      form of SiLU (hardswish): one clamped fma and one
+ *                      multiply per hidden unit instead of an exponential and
+ *                      a reciprocal. */
+#define CFD_ACT_SILU 0
+#define CFD_ACT_HARD_SILU 1
 typedef struct cfd_mlp {   /* row-major [out][in]; 1377 floats in this order */
   float w1[32 * 8];
   float b1[32];
@@ -211,6 +220,7 @@ typedef struct cfd_mlp {   /* row-major [out][in]; 1377 floats in this order */
   float b2[32];
   float w3[32];
   float b3;
+  int32_t activation;      /* CFD_ACT_SILU or CFD_ACT_HARD_SILU            */
 } cfd_mlp;
 
 typedef struct cfd_closure {
@@ -355,8 +365,9 @@ const char *cfd_version(void);
  * of the library it loads and refuses a mismatch, so that no caller passes a
  * struct of another size.  No reference counterpart.
  *   2: cfd_tuning without rotate (library 0.1)
- *   3: cfd_tuning.rotate (library 0.2 and later) */
-#define CFD_ABI_VERSION 3
+ *   3: cfd_tuning.rotate (library 0.2 and 0.3)
+ *   4: cfd_mlp.activation (library 0.4 and later) */
+#define CFD_ABI_VERSION 4
 int cfd_abi_version(void);
 
 #ifdef __cplusplus

@@ -152,13 +152,15 @@ def threads() -> int:
     return lib().orc_threads()
 
 
-def closure_array(params, shift, scale, lo, hi, out_scale):
+def closure_array(params, shift, scale, lo, hi, out_scale, ativacao="silu"):
     """The float32 block orc_closure reads: 1377 MLP parameters (cfd_mlp
-    order), then shift, scale, lo, hi (8 each) and out_scale."""
+    order), then shift, scale, lo, hi (8 each), out_scale and the hidden
+    activation (0 SiLU, 1 hard-SiLU)."""
+    codigo = {"silu": 0.0, "hard_silu": 1.0}[ativacao]
     return np.ascontiguousarray(np.concatenate([
         np.asarray(params, np.float32).reshape(-1), np.asarray(shift, np.float32),
         np.asarray(scale, np.float32), np.asarray(lo, np.float32), np.asarray(hi, np.float32),
-        np.array([out_scale], np.float32)]), dtype=np.float32)
+        np.array([out_scale, codigo], np.float32)]), dtype=np.float32)
 
 
 def closure_eval(cl, X):

@@ -78,16 +78,19 @@ static void tables(void) {
  * features, the weights and each hidden activation are rounded to bf16
  * (round to nearest even); products accumulate in float32 from the bias in
  * increasing input order (the matrix cores sum in another order, so results
- * agree to float32 accumulation error, not bit for bit).  The network is
- * evaluated in its base-2 form, as the matrix-core path: W1, b1 and b2 are
- * multiplied by -log2(e) in float32 (before the bf16 rounding of W1), so
- * layers 1 and 2 give u = -log2(e) z and the SiLU is u / (1 + 2^u) =
- * -log2(e) SiLU(z); W2 is unchanged and w3 is multiplied by -ln 2.  Returns
+ * agree to float32 accumulation error, not bit for bit).  A SiLU network
+ * (activation 0) is evaluated in its base-2 form, as the matrix-core path:
+ * W1, b1 and b2 are multiplied by -log2(e) in float32 (before the bf16
+ * rounding of W1), so layers 1 and 2 give u = -log2(e) z and the SiLU is
+ * u / (1 + 2^u) = -log2(e) SiLU(z); W2 is unchanged and w3 is multiplied by
+ * -ln 2.  A hard-SiLU network (activation 1) is evaluated as it is, each
+ * hidden unit z * clamp(fma(z, 1/6, 1/2), 0, 1).  Returns
  * out_scale * softplus(z), or NaN when a normalized feature leaves
  * [lo, hi] (the caller then keeps the Smagorinsky tau). */
 typedef struct {
   float p[1377];                  /* w1[32][8] b1[32] w2[32][32] b2[32] w3[32] b3 */
   float shift[8], scale[8], lo[8], hi[8], out_scale;
+  float activation;               /* 0 SiLU, 1 hard-SiLU */
 } orc_closure;
 
 static float bf16r(float f) {
@@ -102,10 +105,15 @@ static float bf16r(float f) {
 
 static const float kNegLog2e = -1.44269504f, kNegLn2 = -0.693147181f;
 static float silu2_ref(float u) { return u * (1.0f / (1.0f + exp2f(u))); }
+static float hard_silu_ref(float z) {
+  return z * fminf(fmaxf(fmaf(z, 1.0f / 6.0f, 0.5f), 0.0f), 1.0f);
+}
 
 static float orc_closure_nu(const orc_closure* c, const float x[8]) {
   const float *w1 = c->p, *b1 = c->p + 256, *w2 = c->p + 288, *b2 = c->p + 1312,
               *w3 = c->p + 1344, b3 = c->p[1376];
+  const int hard = c->activation == 1.0f;
+  const float s1 = hard ? 1.0f : kNegLog2e, s3 = hard ? 1.0f : kNegLn2;
   float xb[8], h1[32], h2[32];
   for (int k = 0; k < 8; ++k) {
     const float xn = (x[k] - c->shift[k]) * c->scale[k];
@@ -113,17 +121,17 @@ static float orc_closure_nu(const orc_closure* c, const float x[8]) {
     xb[k] = bf16r(xn);
   }
   for (int o = 0; o < 32; ++o) {
-    float acc = kNegLog2e * b1[o];
-    for (int k = 0; k < 8; ++k) acc += bf16r(kNegLog2e * w1[o * 8 + k]) * xb[k];
-    h1[o] = bf16r(silu2_ref(acc));
+    float acc = s1 * b1[o];
+    for (int k = 0; k < 8; ++k) acc += bf16r(s1 * w1[o * 8 + k]) * xb[k];
+    h1[o] = bf16r(hard ? hard_silu_ref(acc) : silu2_ref(acc));
   }
   for (int o = 0; o < 32; ++o) {
-    float acc = kNegLog2e * b2[o];
+    float acc = s1 * b2[o];
     for (int k = 0; k < 32; ++k) acc += bf16r(w2[o * 32 + k]) * h1[k];
-    h2[o] = bf16r(silu2_ref(acc));
+    h2[o] = bf16r(hard ? hard_silu_ref(acc) : silu2_ref(acc));
   }
   float z = b3;
-  for (int k = 0; k < 32; ++k) z += bf16r(kNegLn2 * w3[k]) * h2[k];
+  for (int k = 0; k < 32; ++k) z += bf16r(s3 * w3[k]) * h2[k];
   const float sp = z > 20.0f ? z : (z < -15.0f ? expf(z) : log1pf(expf(z)));
   return c->out_scale * sp;
 }

@@ -46,6 +46,11 @@ def test_struct_layout_version_matches_the_header():
     assert m and int(m.group(1)) == _lib.ABI_VERSION == _lib.lib.cfd_abi_version()
     assert C.sizeof(_lib.Tuning) == 6 * 4          # cfd_tuning with rotate
     assert C.sizeof(_lib.Params) == 12 * 8
+    assert C.sizeof(_lib.Mlp) == 1378 * 4          # cfd_mlp with activation
+    assert C.sizeof(_lib.Closure) == (1378 + 33) * 4
+    hdr = HEADER.read_text()
+    for nome, v in (("CFD_ACT_SILU", _lib.CFD_ACT_SILU), ("CFD_ACT_HARD_SILU", _lib.CFD_ACT_HARD_SILU)):
+        assert re.search(rf"#define {nome} {v}\b", hdr), nome
     src = (ROOT / "cfd__amd" / "_lib.py").read_text()
     assert "abi != ABI_VERSION" in src
     addon = (ROOT / "cfd__amd" / "node" / "addon.cc").read_text()
@@ -109,6 +114,22 @@ def test_batch_and_closure_argument_errors():
     rc = L.cfd_mlp_train(X.ctypes.data, y.ctypes.data, 64, C.byref(cfg), C.byref(params), None)
     assert rc == -1 and "multiple of 64" in L.cfd_last_error().decode()
     rc = L.cfd_mlp_infer(None, 0, None, None, None)
+    assert rc == -1 and "cfd_mlp_infer" in L.cfd_last_error().decode()
+    # an activation the library does not know is refused by every MLP entry point
+    params.activation = 7
+    cfg.batch = 64
+    rc = L.cfd_mlp_train(X.ctypes.data, y.ctypes.data, 64, C.byref(cfg), C.byref(params), None)
+    assert rc == -1 and "known activation" in L.cfd_last_error().decode()
+    grad = _lib.Mlp()
+    loss = C.c_double()
+    rc = L.cfd_mlp_grad(X.ctypes.data, y.ctypes.data, 64, C.byref(params), C.byref(grad),
+                        C.byref(loss))
+    assert rc == -1 and "cfd_mlp_grad" in L.cfd_last_error().decode()
+    cl = _lib.Closure()
+    cl.mlp = params
+    nu = np.zeros(64, np.float32)
+    inside = np.zeros(64, np.uint8)
+    rc = L.cfd_mlp_infer(X.ctypes.data, 64, C.byref(cl), nu.ctypes.data, inside.ctypes.data)
     assert rc == -1 and "cfd_mlp_infer" in L.cfd_last_error().decode()
     count = C.c_int64()
     assert L.cfd_closure_fallbacks(None, C.byref(count), 0) == -1

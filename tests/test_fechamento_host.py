@@ -32,3 +32,29 @@ def test_wake_field_error():
     only_walls = coarse.copy()
     only_walls.reshape(-1, 3)[:7] += 10.0          # a difference inside walls only
     assert fc.erro_esteira(only_walls, fine, g) < 1e-6
+
+
+def test_model_files_keep_the_activation(tmp_path):
+    """The activation travels with the model: .npz and the Node JSON export,
+    a file without it reads as SiLU, an unknown name is refused."""
+    import json
+
+    import pytest
+
+    from cfd__amd.fechamento import Fechamento, iniciar_mlp
+    rng = np.random.default_rng(4)
+    f = Fechamento(iniciar_mlp(3), rng.normal(size=8).astype(np.float32),
+                   rng.uniform(1, 2, 8).astype(np.float32), np.full(8, -4, np.float32),
+                   np.full(8, 4, np.float32), 1e-3, "hard_silu")
+    f.salvar(tmp_path / "m.npz")
+    g = Fechamento.carregar(tmp_path / "m.npz")
+    assert g.ativacao == "hard_silu" and np.array_equal(g.params, f.params)
+    assert g.c_struct().mlp.activation == 1
+    f.salvarJson(tmp_path / "m.json")
+    assert json.loads((tmp_path / "m.json").read_text())["ativacao"] == "hard_silu"
+    np.savez(tmp_path / "velho.npz", params=f.params, shift=f.shift, scale=f.scale, lo=f.lo,
+             hi=f.hi, out_scale=np.float32(1e-3))
+    velho = Fechamento.carregar(tmp_path / "velho.npz")
+    assert velho.ativacao == "silu" and velho.c_struct().mlp.activation == 0
+    with pytest.raises(ValueError):
+        Fechamento(f.params, f.shift, f.scale, f.lo, f.hi, 1e-3, "tanh").c_struct()

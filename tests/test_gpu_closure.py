@@ -8,6 +8,9 @@ training kernels and the closure step.
 * the closure step, bit for bit, against the plain step where the closure
   must reduce to it (every cell out of range -> Smagorinsky; nu_t = 0 ->
   Cs = 0), and against the oracle's closure step with a random MLP.
+
+Each runs with both hidden activations, SiLU and hard-SiLU
+(z * clamp(z/6 + 1/2, 0, 1); torch's hardswish is the float32 reference).
 """
 import numpy as np
 import pytest
@@ -16,9 +19,10 @@ from cfd__amd import tunel
 from cfd__amd.fechamento import Fechamento, gradiente, iniciar_mlp, treinar_mlp
 
 pytestmark = pytest.mark.gpu
+ATIVACOES = ["silu", "hard_silu"]
 
 
-def _fech(seed=0, out_scale=2e-3, spread=1.0):
+def _fech(seed=0, out_scale=2e-3, spread=1.0, ativacao="silu"):
     rng = np.random.default_rng(seed)
     p = iniciar_mlp(seed, bias_saida=0.3)
     p[1344:1376] *= 10.0   # a non-trivial output layer
@@ -27,33 +31,40 @@ def _fech(seed=0, out_scale=2e-3, spread=1.0):
     shift[7], scale[7] = 5.0, 0.1 * spread       # wall distance, cells
     lo = np.full(8, -60, np.float32)
     hi = np.full(8, 60, np.float32)
-    return Fechamento(p, shift, scale, lo, hi, out_scale)
+    return Fechamento(p, shift, scale, lo, hi, out_scale, ativacao)
 
 
-def _torch_mlp(p, xn, bf16=False):
-    """float32 torch MLP; with bf16, the base-2 form of the matrix-core path
-    (W1, b1, b2 times -log2(e), w3 times -ln 2, SiLU2(u) = u / (1 + 2^u)) with
-    bf16 weights, inputs and hidden activations."""
+def _torch_mlp(p, xn, bf16=False, ativacao="silu"):
+    """float32 torch MLP; with bf16, the form of the matrix-core path with
+    bf16 weights, inputs and hidden activations: for SiLU the base-2 form
+    (W1, b1, b2 times -log2(e), w3 times -ln 2, SiLU2(u) = u / (1 + 2^u)),
+    for hard-SiLU the network as it is."""
     torch = pytest.importorskip("torch")
+    F = torch.nn.functional
     P = torch.tensor(np.asarray(p, np.float32))
     x = torch.tensor(xn, dtype=torch.float32)
+    act = F.silu if ativacao == "silu" else F.hardswish
     if not bf16:
-        h1 = torch.nn.functional.silu(x @ P[0:256].reshape(32, 8).T + P[256:288])
-        h2 = torch.nn.functional.silu(h1 @ P[288:1312].reshape(32, 32).T + P[1312:1344])
-        return torch.nn.functional.softplus(h2 @ P[1344:1376] + P[1376]).numpy()
+        h1 = act(x @ P[0:256].reshape(32, 8).T + P[256:288])
+        h2 = act(h1 @ P[288:1312].reshape(32, 32).T + P[1312:1344])
+        return F.softplus(h2 @ P[1344:1376] + P[1376]).numpy()
     r = lambda t: t.to(torch.bfloat16).to(torch.float32)
-    nl2e, nln2 = -1.44269504, -0.693147181
-    silu2 = lambda u: u / (1.0 + torch.exp2(u))
-    w1 = r(nl2e * P[0:256].reshape(32, 8)); b1 = nl2e * P[256:288]
-    w2 = r(P[288:1312].reshape(32, 32)); b2 = nl2e * P[1312:1344]
-    w3 = r(nln2 * P[1344:1376]); b3 = P[1376]
-    h1 = r(silu2(r(x) @ w1.T + b1))
-    h2 = r(silu2(h1 @ w2.T + b2))
-    return torch.nn.functional.softplus(h2 @ w3 + b3).numpy()
+    if ativacao == "silu":
+        s1, s3 = -1.44269504, -0.693147181
+        act = lambda u: u / (1.0 + torch.exp2(u))
+    else:
+        s1, s3 = 1.0, 1.0
+    w1 = r(s1 * P[0:256].reshape(32, 8)); b1 = s1 * P[256:288]
+    w2 = r(P[288:1312].reshape(32, 32)); b2 = s1 * P[1312:1344]
+    w3 = r(s3 * P[1344:1376]); b3 = P[1376]
+    h1 = r(act(r(x) @ w1.T + b1))
+    h2 = r(act(h1 @ w2.T + b2))
+    return F.softplus(h2 @ w3 + b3).numpy()
 
 
-def test_mlp_inference_matches_oracle_and_torch(orc):
-    fech = _fech(1)
+@pytest.mark.parametrize("ativacao", ATIVACOES)
+def test_mlp_inference_matches_oracle_and_torch(orc, ativacao):
+    fech = _fech(1, ativacao=ativacao)
     rng = np.random.default_rng(2)
     n = 4096 + 37                       # a ragged last wave
     xn = rng.normal(0, 1.5, (n, 8)).astype(np.float32)
@@ -61,7 +72,8 @@ def test_mlp_inference_matches_oracle_and_torch(orc):
     X[5, 3] = 1e3                       # out of the trained range
     nu, dentro = fech.inferir(X)
     assert not dentro[5] and dentro.sum() == n - 1
-    cl = orc.closure_array(fech.params, fech.shift, fech.scale, fech.lo, fech.hi, fech.out_scale)
+    cl = orc.closure_array(fech.params, fech.shift, fech.scale, fech.lo, fech.hi, fech.out_scale,
+                           ativacao)
     ref = orc.closure_eval(cl, X)
     ok = np.isfinite(ref)
     assert ok.sum() == n - 1
@@ -72,24 +84,33 @@ def test_mlp_inference_matches_oracle_and_torch(orc):
     # by at most a bf16 step of one hidden unit
     assert np.median(err) < 1e-5 and err.max() < 5e-3, (np.median(err), err.max())
     # plain float32 torch MLP: the bf16 operand rounding is the only difference
-    t32 = fech.out_scale * _torch_mlp(fech.params, fech.normalizar(X)[ok])
+    t32 = fech.out_scale * _torch_mlp(fech.params, fech.normalizar(X)[ok], ativacao=ativacao)
     assert np.abs(nu[ok] - t32).max() / scale < 5e-2
-    tbf = fech.out_scale * _torch_mlp(fech.params, fech.normalizar(X)[ok], bf16=True)
+    tbf = fech.out_scale * _torch_mlp(fech.params, fech.normalizar(X)[ok], bf16=True,
+                                      ativacao=ativacao)
     assert np.median(np.abs(nu[ok] - tbf) / scale) < 1e-5
+    # and the other activation is another function: its median distance is
+    # a hundred times the tolerance above
+    outra = "hard_silu" if ativacao == "silu" else "silu"
+    t_outra = fech.out_scale * _torch_mlp(fech.params, fech.normalizar(X)[ok], bf16=True,
+                                          ativacao=outra)
+    assert np.median(np.abs(nu[ok] - t_outra) / scale) > 1e-3
 
 
-def test_training_gradient_matches_torch_autograd():
+@pytest.mark.parametrize("ativacao", ATIVACOES)
+def test_training_gradient_matches_torch_autograd(ativacao):
     torch = pytest.importorskip("torch")
     rng = np.random.default_rng(3)
     n = 1000
     Xn = rng.normal(0, 1, (n, 8)).astype(np.float32)
     y = rng.uniform(0, 2, n).astype(np.float32)
     p = iniciar_mlp(4, bias_saida=0.1)
-    g, loss = gradiente(Xn, y, p)
+    g, loss = gradiente(Xn, y, p, ativacao)
     P = torch.tensor(p, requires_grad=True)
     x = torch.tensor(Xn)
-    h1 = torch.nn.functional.silu(x @ P[0:256].reshape(32, 8).T + P[256:288])
-    h2 = torch.nn.functional.silu(h1 @ P[288:1312].reshape(32, 32).T + P[1312:1344])
+    act = torch.nn.functional.silu if ativacao == "silu" else torch.nn.functional.hardswish
+    h1 = act(x @ P[0:256].reshape(32, 8).T + P[256:288])
+    h2 = act(h1 @ P[288:1312].reshape(32, 32).T + P[1312:1344])
     yhat = torch.nn.functional.softplus(h2 @ P[1344:1376] + P[1376])
     L = ((yhat - torch.tensor(y)) ** 2).mean()
     L.backward()
@@ -101,22 +122,23 @@ def test_training_gradient_matches_torch_autograd():
         assert np.abs(gt[a:b]).max() > 0
 
 
-def test_training_fits_a_teacher_network():
+@pytest.mark.parametrize("ativacao", ATIVACOES)
+def test_training_fits_a_teacher_network(ativacao):
     rng = np.random.default_rng(5)
     n = 20000
     Xn = rng.normal(0, 1, (n, 8)).astype(np.float32)
     teacher = iniciar_mlp(11, bias_saida=0.5)
-    y = _torch_mlp(teacher, Xn).astype(np.float32)
+    y = _torch_mlp(teacher, Xn, ativacao=ativacao).astype(np.float32)
     p0 = iniciar_mlp(12)
-    _, l0 = gradiente(Xn, y, p0)
-    p, curva = treinar_mlp(Xn, y, p0, passos=3000, lote=1024, lr=1e-3, seed=7)
-    _, l1 = gradiente(Xn, y, p)
+    _, l0 = gradiente(Xn, y, p0, ativacao)
+    p, curva = treinar_mlp(Xn, y, p0, passos=3000, lote=1024, lr=1e-3, seed=7, ativacao=ativacao)
+    _, l1 = gradiente(Xn, y, p, ativacao)
     assert curva.shape == (3000,) and np.isfinite(curva).all()
     assert l1 < 0.03 * l0, (l0, l1)
     assert curva[-100:].mean() < 0.1 * curva[:100].mean()
     # deterministic: the same seed gives the same parameters
-    p2, _ = treinar_mlp(Xn, y, p0, passos=50, lote=1024, lr=1e-3, seed=7)
-    p3, _ = treinar_mlp(Xn, y, p0, passos=50, lote=1024, lr=1e-3, seed=7)
+    p2, _ = treinar_mlp(Xn, y, p0, passos=50, lote=1024, lr=1e-3, seed=7, ativacao=ativacao)
+    p3, _ = treinar_mlp(Xn, y, p0, passos=50, lote=1024, lr=1e-3, seed=7, ativacao=ativacao)
     assert np.array_equal(p2, p3)
 
 
@@ -161,12 +183,13 @@ def _run(g, tipo, esponja, g0, n, fech=None, wd=None, precisao="fp32", lesCs=0.1
     return out, fb
 
 
-@pytest.mark.parametrize("precisao", ["fp32", "fp16"])
-def test_closure_out_of_range_is_smagorinsky_bit_for_bit(precisao):
+@pytest.mark.parametrize("precisao,ativacao", [("fp32", "silu"), ("fp16", "silu"),
+                                               ("fp16", "hard_silu")])
+def test_closure_out_of_range_is_smagorinsky_bit_for_bit(precisao, ativacao):
     from cfd__amd.fechamento import distancia_parede
     g, tipo, esp = _tunnel()
     g0 = _state(g, 1)
-    fech = _fech(2)
+    fech = _fech(2, ativacao=ativacao)
     fech.lo[:] = 1e30
     fech.hi[:] = 1e30                  # no cell is inside: every fluid cell falls back
     wd = distancia_parede(tipo, g)
@@ -178,11 +201,12 @@ def test_closure_out_of_range_is_smagorinsky_bit_for_bit(precisao):
     assert fb == 7 * fluid
 
 
-def test_closure_with_zero_viscosity_is_the_cs0_step_bit_for_bit():
+@pytest.mark.parametrize("ativacao", ATIVACOES)
+def test_closure_with_zero_viscosity_is_the_cs0_step_bit_for_bit(ativacao):
     from cfd__amd.fechamento import distancia_parede
     g, tipo, esp = _tunnel()
     g0 = _state(g, 3)
-    fech = _fech(4, out_scale=0.0)
+    fech = _fech(4, out_scale=0.0, ativacao=ativacao)
     fech.lo[:] = -np.inf
     fech.hi[:] = np.inf
     wd = distancia_parede(tipo, g)
@@ -193,8 +217,9 @@ def test_closure_with_zero_viscosity_is_the_cs0_step_bit_for_bit():
     assert np.array_equal(a[:, ~wall], b[:, ~wall])
 
 
+@pytest.mark.parametrize("ativacao", ATIVACOES)
 @pytest.mark.parametrize("precisao", ["fp32exact", "fp32"])
-def test_closure_step_matches_oracle(orc, precisao):
+def test_closure_step_matches_oracle(orc, precisao, ativacao):
     """One step on identical inputs (the initial state, and the GPU's own
     state after 9 steps) against one oracle step, and the 10-step trajectory.
 
@@ -210,12 +235,13 @@ def test_closure_step_matches_oracle(orc, precisao):
     g, tipo, esp = _tunnel()
     grid = (g["nx"], g["ny"], g["nz"])
     g0 = _state(g, 5)
-    fech = _fech(6, out_scale=4e-3, spread=0.5)
+    fech = _fech(6, out_scale=4e-3, spread=0.5, ativacao=ativacao)
     wd = distancia_parede(tipo, g)
     wdq = np.minimum(np.floor(wd * 8 + 0.5), 255).astype(np.float32) / 8   # the device's 1/8 cell
     par = orc.params_array(1.8, lesCs=0.1, inletU=[0.05, 0, 0], beltU=[0.05, 0, 0],
                            esponja=esp, espessuraCL=4.0)
-    cl = orc.closure_array(fech.params, fech.shift, fech.scale, fech.lo, fech.hi, fech.out_scale)
+    cl = orc.closure_array(fech.params, fech.shift, fech.scale, fech.lo, fech.hi, fech.out_scale,
+                           ativacao)
     wall = np.isin(tipo, [1, 4, 7])
     exact = precisao == "fp32exact"
 
@@ -355,8 +381,9 @@ def test_wall_distance_follows_the_types_unless_given():
         s.destruir()
 
 
-def test_c5_closure_at_size(orc):
-    """The learned-closure workload itself: the shipped model on the coarse
+@pytest.mark.parametrize("modelo", ["MODELO", "MODELO_HARD_SILU"])
+def test_c5_closure_at_size(orc, modelo):
+    """The learned-closure workload itself: each shipped model on the coarse
     240x120x96 vehicle tunnel (seed 7 at 0.75 scale, no-slip floor, inlet
     boundary layer, sponge; omega 1.90, Cs 0.1).  From a flow developed by
     1500 plain steps, two closure steps on the GPU agree with the oracle's
@@ -372,7 +399,7 @@ def test_c5_closure_at_size(orc):
     caso = fc.montar(fc.AVALIACAO_SEED, 1)
     g, tipo = caso["grade"], caso["tipo"]
     grid = (g["nx"], g["ny"], g["nz"])
-    fech = Fechamento.carregar(fc.MODELO)
+    fech = Fechamento.carregar(getattr(fc, modelo))
     cfg = dict(omegaPlus=fc.omega_de(fc.NU_C), lesCs=fc.CS, esponja=caso["esponja"],
                inletU=[fc.U, 0.0, 0.0], beltU=[0.0, 0.0, 0.0],
                espessuraCL=float(np.floor(g["nz"] * 0.12 + 0.5)))
@@ -398,7 +425,8 @@ def test_c5_closure_at_size(orc):
         s.destruir()
     assert fb <= 0.1 * 2 * fluid, fb / (2 * fluid)
     par = orc.params_array(nx=grid[0], **cfg)
-    cl = orc.closure_array(fech.params, fech.shift, fech.scale, fech.lo, fech.hi, fech.out_scale)
+    cl = orc.closure_array(fech.params, fech.shift, fech.scale, fech.lo, fech.hi, fech.out_scale,
+                           fech.ativacao)
     ref = g0.reshape(-1).copy()
     for k in range(2):
         ref, _ = orc.step_closure(grid, ref, tipo, par, cl, wdq)

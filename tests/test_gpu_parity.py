@@ -333,14 +333,15 @@ def test_layouts_and_launch_shapes_are_bit_identical():
         assert np.array_equal(m, ref_m), k
 
 
-@pytest.mark.parametrize("precisao,fechamento", [("fp16", False), ("fp16sr", False),
-                                                  ("fp16sr", True), ("fp32", True)])
+@pytest.mark.parametrize("precisao,fechamento", [("fp16", None), ("fp16sr", None),
+                                                  ("fp16sr", "silu"), ("fp32", "silu"),
+                                                  ("fp16sr", "hard_silu")])
 def test_float16_and_closure_layouts_are_bit_identical(precisao, fechamento):
     """The float16 steps (stored units, paired conversions, stochastic rounding
-    drawn from the cell, direction and step only) and the closure step give
-    the same bits in the SoA layout (the default for float16 sets up to
-    128 MB) and in 512-cell blocks, in several launch shapes of the plain
-    step."""
+    drawn from the cell, direction and step only) and the closure step (SiLU
+    and hard-SiLU networks) give the same bits in the SoA layout (the default
+    for float16 sets up to 128 MB) and in 512-cell blocks, in several launch
+    shapes of the plain step."""
     from cfd__amd.fechamento import Fechamento, distancia_parede, iniciar_mlp
     grid = (72, 40, 36)
     nx, ny, nz = grid
@@ -357,7 +358,7 @@ def test_float16_and_closure_layouts_are_bit_identical(precisao, fechamento):
         scale = (1.0 / rng.uniform(5e-4, 2e-3, 8)).astype(np.float32)
         shift[7], scale[7] = 5.0, 0.1
         fech = Fechamento(prm, shift, scale, np.full(8, -60, np.float32),
-                          np.full(8, 60, np.float32), 2e-3)
+                          np.full(8, 60, np.float32), 2e-3, fechamento)
         wd = distancia_parede(tipo, g)
     shapes = ({"layout": 0, "block": 256, "sweep": 0}, {"layout": 1, "block": 256, "sweep": 0},
               {"layout": 1, "block": 512, "sweep": 1}, {"layout": 0, "block": 512, "sweep": 0,

@@ -139,15 +139,17 @@ def test_node_validation_page_matches_python(tmp_path):
 
 
 @pytest.mark.gpu
-def test_node_closure_matches_python(tmp_path):
-    """The learned closure from Node (the shipped model through its JSON
+@pytest.mark.parametrize("modelo", ["fechamento_c5.npz", "fechamento_c5_hard_silu.npz"])
+def test_node_closure_matches_python(tmp_path, modelo):
+    """The learned closure from Node (each shipped model through its JSON
     export, wall distance on the GPU, float16 storage with stochastic
     rounding) gives the Python run's populations bit for bit, and the
     closure is in use: the result differs from the Smagorinsky step."""
     from cfd__amd import lattice
     from cfd__amd.fechamento import Fechamento
-    from cfd__amd.fechamento_caso import MODELO
+    from cfd__amd.fechamento_caso import MODELO as PADRAO
     from cfd__amd.solver import SolverMI355X
+    MODELO = PADRAO.with_name(modelo)
     nx, ny, nz = 40, 24, 20
     g = {"nx": nx, "ny": ny, "nz": nz}
     corpo = tunel.corpoAnalitico(g, {"tipo": "esfera", "cx": 14, "cy": 12, "cz": 9, "raio": 4.5})

@@ -354,31 +354,40 @@ def _bf16(a):
 NL2E, NLN2 = np.float32(-1.44269504), np.float32(-0.693147181)
 
 
-def _mlp_numpy(p, xn, bf16=True):
-    """Independent float64 evaluation of the closure MLP in its base-2 form:
-    W1, b1 and b2 times -log2(e) and w3 times -ln 2, each product in float32;
-    with bf16, weights, inputs and hidden activations rounded to bf16 (the
-    oracle sums in float32).  SiLU2(u) = u / (1 + 2^u)."""
+def _hard_silu64(a):
+    """z * clamp(z/6 + 1/2, 0, 1) in float64 (Howard et al. 2019, 'hardswish')."""
+    return a * np.clip(a / 6.0 + 0.5, 0.0, 1.0)
+
+
+def _mlp_numpy(p, xn, bf16=True, ativacao="silu"):
+    """Independent float64 evaluation of the closure MLP as the matrix-core
+    path evaluates it.  SiLU: the base-2 form, W1, b1 and b2 times -log2(e)
+    and w3 times -ln 2, each product in float32, SiLU2(u) = u / (1 + 2^u).
+    Hard-SiLU: the network as it is.  With bf16, weights, inputs and hidden
+    activations rounded to bf16 (the oracle sums in float32)."""
     r = _bf16 if bf16 else (lambda a: np.asarray(a, np.float32))
-    silu2 = lambda u: u / (1.0 + np.exp2(u))
-    w1 = r(NL2E * p[0:256]).reshape(32, 8).astype(np.float64)
+    if ativacao == "silu":
+        act, s1, s3 = (lambda u: u / (1.0 + np.exp2(u))), NL2E, NLN2
+    else:
+        act, s1, s3 = _hard_silu64, np.float32(1), np.float32(1)
+    w1 = r(s1 * p[0:256]).reshape(32, 8).astype(np.float64)
     w2 = r(p[288:1312]).reshape(32, 32).astype(np.float64)
-    w3 = r(NLN2 * p[1344:1376]).astype(np.float64)
+    w3 = r(s3 * p[1344:1376]).astype(np.float64)
     x = r(xn).astype(np.float64)
-    a1 = x @ w1.T + (NL2E * p[256:288]).astype(np.float64)
-    h1 = r(silu2(a1).astype(np.float32)).astype(np.float64)
-    a2 = h1 @ w2.T + (NL2E * p[1312:1344]).astype(np.float64)
-    h2 = r(silu2(a2).astype(np.float32)).astype(np.float64)
+    a1 = x @ w1.T + (s1 * p[256:288]).astype(np.float64)
+    h1 = r(act(a1).astype(np.float32)).astype(np.float64)
+    a2 = h1 @ w2.T + (s1 * p[1312:1344]).astype(np.float64)
+    h2 = r(act(a2).astype(np.float32)).astype(np.float64)
     z = h2 @ w3 + p[1376]
     return np.logaddexp(0.0, z)
 
 
-def _mlp_plain64(p, xn):
+def _mlp_plain64(p, xn, ativacao="silu"):
     """The same network in its usual form, float64 throughout (no rounding)."""
     p = p.astype(np.float64)
-    silu = lambda a: a / (1.0 + np.exp(-a))
-    h1 = silu(xn.astype(np.float64) @ p[0:256].reshape(32, 8).T + p[256:288])
-    h2 = silu(h1 @ p[288:1312].reshape(32, 32).T + p[1312:1344])
+    act = (lambda a: a / (1.0 + np.exp(-a))) if ativacao == "silu" else _hard_silu64
+    h1 = act(xn.astype(np.float64) @ p[0:256].reshape(32, 8).T + p[256:288])
+    h2 = act(h1 @ p[288:1312].reshape(32, 32).T + p[1312:1344])
     return np.logaddexp(0.0, h2 @ p[1344:1376] + p[1376])
 
 
@@ -398,19 +407,20 @@ def test_base2_form_is_the_same_network():
     assert np.abs(_mlp_plain64(q, xn) - a).max() > 1e-2 * np.abs(b).max()
 
 
-def test_oracle_closure_mlp_against_independent_numpy(orc):
+@pytest.mark.parametrize("ativacao", ["silu", "hard_silu"])
+def test_oracle_closure_mlp_against_independent_numpy(orc, ativacao):
     rng = np.random.default_rng(21)
     p = rng.normal(0, 0.4, 1377).astype(np.float32)
     shift = rng.normal(0, 1, 8).astype(np.float32)
     scale = rng.uniform(0.5, 2, 8).astype(np.float32)
     lo, hi = np.full(8, -5, np.float32), np.full(8, 5, np.float32)
     X = rng.normal(0, 1.5, (3000, 8)).astype(np.float32)
-    cl = orc.closure_array(p, shift, scale, lo, hi, 0.7)
+    cl = orc.closure_array(p, shift, scale, lo, hi, 0.7, ativacao)
     nu = orc.closure_eval(cl, X)
     xn = ((X - shift) * scale).astype(np.float32)
     inside = np.all((xn >= lo) & (xn <= hi), axis=1)
     assert np.array_equal(np.isfinite(nu), inside) and 0 < inside.sum() < 3000
-    ref = 0.7 * _mlp_numpy(p, xn[inside])
+    ref = 0.7 * _mlp_numpy(p, xn[inside], ativacao=ativacao)
     err = np.abs(nu[inside] - ref) / np.abs(ref).max()
     # float32 vs float64 sums: a few hidden activations land on the other
     # bf16 neighbour; the bulk agrees to float32 precision
@@ -418,7 +428,52 @@ def test_oracle_closure_mlp_against_independent_numpy(orc):
     # a dropped layer or a transposed weight matrix would not come close
     pt = p.copy()
     pt[288:1312] = p[288:1312].reshape(32, 32).T.reshape(-1)
-    assert np.abs(0.7 * _mlp_numpy(pt, xn[inside]) - ref).max() > 1e-2 * np.abs(ref).max()
+    assert np.abs(0.7 * _mlp_numpy(pt, xn[inside], ativacao=ativacao) - ref).max() > (
+        1e-2 * np.abs(ref).max())
+    # nor would the other activation: the oracle evaluates the one it is given
+    outra = "hard_silu" if ativacao == "silu" else "silu"
+    assert np.abs(0.7 * _mlp_numpy(p, xn[inside], ativacao=outra) - ref).max() > (
+        1e-2 * np.abs(ref).max())
+
+
+def test_hard_silu_form_is_the_same_network():
+    """Without the bf16 roundings the hard-SiLU evaluation is the usual
+    network to float32 rounding, and hard-SiLU is not SiLU."""
+    rng = np.random.default_rng(6)
+    p = rng.normal(0, 0.4, 1377).astype(np.float32)
+    xn = rng.normal(0, 1.5, (500, 8)).astype(np.float32)
+    a = _mlp_numpy(p, xn, bf16=False, ativacao="hard_silu")
+    b = _mlp_plain64(p, xn, ativacao="hard_silu")
+    assert np.abs(a - b).max() < 1e-5 * np.abs(b).max()
+    assert np.abs(_mlp_plain64(p, xn) - b).max() > 1e-2 * np.abs(b).max()
+
+
+def test_oracle_hard_silu_closed_form(orc):
+    """A chain through one hidden unit per layer: z = h(h(x0)) + b3 with
+    h(z) = z * clamp(z/6 + 1/2, 0, 1), every other unit at h(0) = 0.  The
+    inputs and every intermediate value are exact in bf16, so the oracle
+    returns out_scale * softplus(z) to float32 rounding: h(1.5) = 1.125,
+    h(1.125) = 0.7734375; h(-1.5) = -0.375, h(-0.375) = -0.1640625;
+    saturation h(4) = 4, h(-4) = 0; h(0.75) = 0.46875, h(0.46875) = 0.27099609375
+    -> 0.271484375 in bf16.  A slope of 1/3 or an offset of 1/4 in the clamp fails."""
+    p = np.zeros(1377, np.float32)
+    p[0] = 1.0            # w1[0][0]: unit 0 of layer 1 = x0
+    p[288] = 1.0          # w2[0][0]: unit 0 of layer 2 = h1[0]
+    p[1344] = 1.0         # w3[0]
+    p[1376] = 0.25        # b3
+    x0 = np.array([1.5, -1.5, 4.0, -4.0, 0.75, 0.0], np.float32)
+    h2 = np.array([0.7734375, -0.1640625, 4.0, 0.0, 0.271484375, 0.0])
+    X = np.zeros((x0.size, 8), np.float32)
+    X[:, 0] = x0
+    cl = orc.closure_array(p, np.zeros(8), np.ones(8), np.full(8, -10), np.full(8, 10), 0.5,
+                           "hard_silu")
+    nu = orc.closure_eval(cl, X)
+    ref = 0.5 * np.logaddexp(0.0, h2 + 0.25)
+    assert np.abs(nu - ref).max() <= 1e-6 * np.abs(ref).max(), (nu, ref)
+    for slope, off in ((1 / 3, 0.5), (1 / 6, 0.25)):
+        h = lambda z: z * np.clip(z * slope + off, 0, 1)
+        wrong = 0.5 * np.logaddexp(0.0, h(h(x0.astype(np.float64))) + 0.25)
+        assert np.abs(wrong - ref).max() > 1e-3
 
 
 def test_oracle_closure_step_reduces_to_the_plain_step(orc):
