@@ -188,9 +188,12 @@ int cfd_batch_create(const cfd_tunnel_spec* specs, int n, cfd_batch** out) {
                         "cfd_batch_create: tunnel %d: %s", k, hipGetErrorString(e)));
     }
   }
+  // the descriptors on the batch's stream (non-blocking: a null-stream copy
+  // would not order before the runs that read them)
   if (hipMalloc(&b->dev, sizeof(BatchTunnel) * n) != hipSuccess ||
-      hipMemcpy(b->dev, b->host.data(), sizeof(BatchTunnel) * n, hipMemcpyHostToDevice) !=
-          hipSuccess)
+      hipMemcpyAsync(b->dev, b->host.data(), sizeof(BatchTunnel) * n, hipMemcpyHostToDevice,
+                     b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
     return bail(bfail(CFD_E_HIP, "cfd_batch_create: descriptor upload failed"));
   *out = b;
   return CFD_OK;

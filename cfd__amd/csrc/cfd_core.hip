@@ -49,7 +49,7 @@ struct cfd_solver {
   Store store = Store::F32;
   size_t pop_bytes = 0;  // bytes of one population set
   Geom geom{};
-  Tuning tuning{256, 8, kLayoutBlocked, 0, 0, 0};  // measured best on the 6.55 M-cell vehicle
+  Tuning tuning{256, 8, kLayoutBlocked, 0, 0, 0};  // cfd_create adjusts it per population set
   StepParams params{};
   bool configured = false;
   int front = 0;  // 0 = A, 1 = B
@@ -713,7 +713,10 @@ int cfd_set_closure(cfd_solver* s, const cfd_closure* c) {
   MlpPacked pk;
   pack_closure(*c, &pk);
   if (!s->mlp) HIPCK(hipMalloc(&s->mlp, sizeof(MlpPacked)));
-  HIPCK(hipMemcpy(s->mlp, &pk, sizeof pk, hipMemcpyHostToDevice));
+  // on the handle's stream (non-blocking: a null-stream copy would not order
+  // before the steps that read it), complete before pk goes out of scope
+  HIPCK(hipMemcpyAsync(s->mlp, &pk, sizeof pk, hipMemcpyHostToDevice, s->stream));
+  HIPCK(hipStreamSynchronize(s->stream));
   s->mlp_act = c->mlp.activation;
   return CFD_OK;
 }
@@ -727,7 +730,9 @@ int cfd_set_wall_distance(cfd_solver* s, const float* d, size_t n) {
   if (rc) return rc;
   std::vector<uint8_t> q(n);
   for (size_t k = 0; k < n; ++k) q[k] = quantize_wall(d[k]);
-  HIPCK(hipMemcpy(s->wall8, q.data(), n, hipMemcpyHostToDevice));
+  // on the handle's stream: steps already enqueued read the previous field
+  HIPCK(hipMemcpyAsync(s->wall8, q.data(), n, hipMemcpyHostToDevice, s->stream));
+  HIPCK(hipStreamSynchronize(s->stream));
   s->wall_given = true;
   return CFD_OK;
 }
@@ -761,7 +766,8 @@ int cfd_closure_fallbacks(cfd_solver* s, int64_t* count, int reset) {
   unsigned long long v[kFallbackSlots];
   HIPCK(hipStreamSynchronize(s->stream));
   HIPCK(hipMemcpy(v, s->fallbacks, sizeof v, hipMemcpyDeviceToHost));
-  if (reset) HIPCK(hipMemset(s->fallbacks, 0, sizeof v));
+  // the reset on the handle's stream, so that it lands before the next step
+  if (reset) HIPCK(hipMemsetAsync(s->fallbacks, 0, sizeof v, s->stream));
   unsigned long long sum = 0;
   for (int k = 0; k < kFallbackSlots; ++k) sum += v[k];
   *count = (int64_t)sum;

@@ -381,6 +381,61 @@ def test_wall_distance_follows_the_types_unless_given():
         s.destruir()
 
 
+def test_closure_uploads_are_ordered_after_enqueued_steps():
+    """A new wall distance uploaded while closure steps are still queued on
+    the handle's stream applies after them: the run equals the same sequence
+    with a read-back (a full synchronization) before the upload.  A counter
+    reset lands before the steps that follow it: with every cell out of range,
+    k steps after a reset count exactly k times the fluid cells."""
+    from cfd__amd.fechamento import distancia_parede
+    from cfd__amd.solver import SolverMI355X
+    g, tipo, esp = _tunnel((64, 40, 36))
+    g0 = _state(g, 8)
+    wd = distancia_parede(tipo, g)
+    outra = np.where(wd > 0, wd * 0.5 + 1.0, 0.0).astype(np.float32)
+
+    def run(sincronizar):
+        s = SolverMI355X.criar(g, precisao="fp16sr")
+        try:
+            s.definirTipos(tipo)
+            s.configurar(omegaPlus=1.8, lesCs=0.1, inletU=[0.05, 0, 0], beltU=[0.05, 0, 0],
+                         esponja=esp, espessuraCL=4.0)
+            s.frente = "A"
+            s.definirPopulacoes(g0)
+            s.definirDistanciaParede(wd)
+            s.definirFechamento(_fech(6, out_scale=4e-3, spread=0.5))
+            s.passo(300)
+            if sincronizar:
+                s.lerPopulacoes()
+            s.definirDistanciaParede(outra)
+            s.passo(20)
+            return s.lerPopulacoes()
+        finally:
+            s.destruir()
+    wall = np.isin(tipo, [1, 4, 7])
+    a, b = run(False), run(True)
+    assert np.array_equal(a[:, ~wall], b[:, ~wall])
+
+    s = SolverMI355X.criar(g, precisao="fp16")
+    try:
+        s.definirTipos(tipo)
+        s.configurar(omegaPlus=1.8, lesCs=0.1, inletU=[0.05, 0, 0], beltU=[0.05, 0, 0],
+                     esponja=esp, espessuraCL=4.0)
+        s.inicializar()
+        fech = _fech(2)
+        fech.lo[:] = 1e30
+        fech.hi[:] = 1e30
+        s.definirFechamento(fech)
+        fluid = int(np.sum(~wall & ~np.isin(tipo, [2, 3, 5, 6])))
+        for k in (40, 3, 17):
+            s.passo(50)
+            s.fallbacksFechamento(zerar=True)
+            s.passo(k)
+            assert s.fallbacksFechamento(zerar=False) == k * fluid
+    finally:
+        s.destruir()
+
+
 @pytest.mark.parametrize("modelo", ["MODELO", "MODELO_HARD_SILU"])
 def test_c5_closure_at_size(orc, modelo):
     """The learned-closure workload itself: each shipped model on the coarse
