@@ -210,8 +210,10 @@ class SolverMI355X {
       addon.setClosure(this._h, null);
       return;
     }
-    const codigo = ATIVACOES[modelo.ativacao === undefined ? 'silu' : modelo.ativacao];
-    if (codigo === undefined) throw new Error(`fechamento: ativacao desconhecida ${modelo.ativacao}`);
+    const nomeAtivacao = modelo.ativacao === undefined ? 'silu' : modelo.ativacao;
+    if (!Object.prototype.hasOwnProperty.call(ATIVACOES, nomeAtivacao))
+      throw new Error(`fechamento: ativacao desconhecida ${nomeAtivacao}`);
+    const codigo = ATIVACOES[nomeAtivacao];
     const c = new Float32Array(1411);
     const partes = [['params', 1377], ['shift', 8], ['scale', 8], ['lo', 8], ['hi', 8]];
     let k = 0;

@@ -45,6 +45,30 @@ def test_errors_become_js_errors():
     assert out.strip().startswith("true cfd_create: grid 0x4x4 must be positive")
 
 
+def test_closure_model_checks_before_the_device(tmp_path):
+    """The Node closure path reads each shipped model's activation from its
+    JSON (SiLU when the field is absent) and refuses an unknown activation
+    (an inherited property name included) before any device call; the addon
+    refuses a closure without a solver handle."""
+    out = _node(r"""
+const { SolverMI355X, carregarFechamento, addon } = require('./cfd__amd/node/solver.js');
+const a = carregarFechamento('cfd__amd/modelos/fechamento_c5.json');
+const b = carregarFechamento('cfd__amd/modelos/fechamento_c5_hard_silu.json');
+const res = [a.ativacao, b.ativacao];
+for (const nome of ['tanh', 'toString']) {
+  try { SolverMI355X.prototype.definirFechamento.call({ _h: null }, { ...a, ativacao: nome }); res.push('no error'); }
+  catch (e) { res.push(e.message.includes('ativacao desconhecida') ? 'refused' : e.message); }
+}
+try { addon.setClosure(null, new Float32Array(1410)); res.push('no error'); }
+catch (e) { res.push(e.constructor.name); }
+console.log(JSON.stringify(res));
+""")
+    res = json.loads(out.strip().splitlines()[-1])
+    assert res[:2] == ["silu", "hard_silu"]
+    assert res[2:4] == ["refused", "refused"]
+    assert res[4] != "no error"
+
+
 @pytest.mark.gpu
 def test_node_run_matches_python_run(tmp_path):
     """Same case through the JS class and through the Python mirror: the
