@@ -37,7 +37,7 @@ from types import SimpleNamespace
 import numpy as np
 
 from . import meshes, tunel
-from .fechamento import Fechamento, distancia_parede
+from .fechamento import Fechamento, distancia_parede, iniciar_mlp
 from .forces import JanelaCd, coeficientes
 from .lote_casos import strouhal_periodos
 from .voxel import voxelizar
@@ -436,6 +436,10 @@ def main(argv=None):
     ap.add_argument("--modelo", default=None, help="where to save the trained closure (.npz)")
     ap.add_argument("--ativacao", default="silu", choices=["silu", "hard_silu"],
                     help="hidden activation of a newly trained closure")
+    ap.add_argument("--destilar", default=None,
+                    help="train the new closure on this closure's outputs (.npz) on the same "
+                         "features instead of on nu*: the new network, with the teacher's "
+                         "normalization and output scale, learns the teacher's function")
     ap.add_argument("--calibrar", action="store_true",
                     help="choose the output scale by coarse-vs-fine Cd on training shapes")
     ap.add_argument("--calibrar-esteira", action="store_true",
@@ -525,12 +529,26 @@ def main(argv=None):
     X, y, info = gerar_dados(TREINO_SEEDS[:a.formas], a.passos_finos)
     print(f"[fechamento] {X.shape[0]} samples in {time.perf_counter() - t0:.1f} s; nu* mean "
           f"{y.mean():.3e}, p99 {np.percentile(y, 99):.3e}", flush=True)
-    fech = Fechamento.novo(X, y, ativacao=a.ativacao)
+    alvo, destilacao = y, None
+    if a.destilar:
+        prof = Fechamento.carregar(a.destilar)
+        alvo, dentro_prof = prof.inferir(X)
+        assert dentro_prof.all(), "the teacher's range must cover the training features"
+        fech = Fechamento(iniciar_mlp(0), prof.shift, prof.scale, prof.lo, prof.hi,
+                          prof.out_scale, a.ativacao)
+    else:
+        fech = Fechamento.novo(X, y, ativacao=a.ativacao)
     t1 = time.perf_counter()
-    curva = fech.treinar(X, y, passos=a.treino, lote=4096, lr=1e-3)
+    curva = fech.treinar(X, alvo, passos=a.treino, lote=4096, lr=1e-3)
     print(f"[fechamento] trained {a.treino} Adam steps in {time.perf_counter() - t1:.1f} s, "
           f"loss {curva[:100].mean():.4f} -> {curva[-100:].mean():.4f}", flush=True)
     nu_pred, dentro = fech.inferir(X)
+    if a.destilar:
+        destilacao = {"professor": str(a.destilar),
+                      "r2_vs_professor": float(1.0 - np.mean((nu_pred - alvo) ** 2) / np.var(alvo)),
+                      "erro_rel_medio_vs_professor": float(np.mean(np.abs(nu_pred - alvo))
+                                                           / np.mean(np.abs(alvo)))}
+        print(f"[fechamento] distilled from {a.destilar}: {destilacao}", flush=True)
     cal = calibrar(fech) if a.calibrar else None
     if a.modelo:
         fech.salvar(a.modelo)
@@ -541,7 +559,7 @@ def main(argv=None):
                      "dentro": float(dentro.mean()),
                      "perda_inicio": float(curva[:100].mean()),
                      "perda_fim": float(curva[-100:].mean()),
-                     "nu_star_medio": float(y.mean())}
+                     "nu_star_medio": float(y.mean()), "destilacao": destilacao}
     if a.saida:
         with open(a.saida, "w") as fh:
             json.dump(res, fh, indent=1, default=float)
