@@ -277,9 +277,10 @@ def cpu_baseline(case, budget_s: float):
                               "kind": "reference",
                               "sample": "the reference's own voxelizar() (src/geom/voxel.js, "
                                         "Node 12, one thread) on this 200k-triangle vehicle "
-                                        "at 320x160x128, timed in the build container when "
-                                        "tests/golden/voxel_golden.json was written (the "
-                                        "reference does not travel to the GPU box)"}}}
+                                        "at 320x160x128, timed when "
+                                        "tests/golden/voxel_golden.json was written, where "
+                                        "the reference's sources were present (they are "
+                                        "not needed, and not read, at run time)"}}}
 
 
 def _r1(x):
