@@ -23,7 +23,10 @@ usage: python -m cfd__amd.perfil [--repeticoes R] [--precisao fp32|fp16] [--copi
 Prints one JSON object.  Run it under rocprofv3 --kernel-trace --stats for
 per-kernel averages, then
   python -m cfd__amd.perfil --resumo TRACE --json OUT.json [--fetch D --write D]
-      --comando "..." --saida profiles/x.md
+      --comando "PROGRAM" --saida profiles/x.md
+
+where PROGRAM is the profiled command (python3 -m cfd__amd.perfil --repeticoes 20 --copia);
+the summary lists every command of the recipe.
 writes the markdown summary (kernel table, algorithmic bytes per call and
 GB/s from the kernel-trace averages, HBM traffic from the counters).
 """
@@ -105,7 +108,13 @@ def resumo(a) -> None:
                 "k_wd_planes": 3}
     lines = [f"# Kernels around the step, {N // (160 * 128)}x160x128 vehicle "
              f"({N / 1e6:.2f} M cells, {j['triangles']} triangles, {j['precisao']})", "",
-             "Commands (run from the repository root on one MI355X):", "", "```", a.comando,
+             "Commands (run from the repository root on one MI355X):", "", "```",
+             "python3 -m cfd__amd.perfil > perfil.json",
+             f"rocprofv3 --kernel-trace --stats -d <trace> -o run --output-format csv -- {a.comando}",
+             f"rocprofv3 --pmc FETCH_SIZE -d <fetch> -o run --output-format csv -- {a.comando}",
+             f"rocprofv3 --pmc WRITE_SIZE -d <write> -o run --output-format csv -- {a.comando}",
+             "python -m cfd__amd.perfil --resumo <trace> --json perfil.json --fetch <fetch> "
+             f"--write <write> --comando '{a.comando}' --saida {a.saida}",
              "```", "", "## Kernel trace (--stats)", "",
              "| kernel | calls | avg us | algorithmic bytes / call | GB/s | of 8 TB/s |",
              "|---|---|---|---|---|---|"]
