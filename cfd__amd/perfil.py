@@ -24,11 +24,10 @@ Prints one JSON object.  Run it under rocprofv3 --kernel-trace --stats for
 per-kernel averages, then
   python -m cfd__amd.perfil --resumo TRACE --json OUT.json [--fetch D --write D]
       --comando "PROGRAM" --saida profiles/x.md
-
-where PROGRAM is the profiled command (python3 -m cfd__amd.perfil --repeticoes 20 --copia);
-the summary lists every command of the recipe.
 writes the markdown summary (kernel table, algorithmic bytes per call and
-GB/s from the kernel-trace averages, HBM traffic from the counters).
+GB/s from the kernel-trace averages, HBM traffic from the counters).  PROGRAM
+is the profiled command (python3 -m cfd__amd.perfil --repeticoes 20 --copia);
+the summary lists every command of the recipe.
 """
 from __future__ import annotations
 
