@@ -315,8 +315,11 @@ def calibrar_esteira(fech: Fechamento, seeds=TREINO_SEEDS[:3],
             "fator": melhor["fator"], "out_scale": fech.out_scale}
 
 
-# Ramp-length offsets of the realizations of an ensemble (coarse steps).
-RAMPA_CONJUNTO = (0, 37, 74, 111, 148, 185, 222, 259)
+# Ramp-length offsets of the realizations of an ensemble (coarse steps): an
+# ensemble of k members uses the first k, so a smaller ensemble is a subset of
+# a larger one; the longest ramp still ends 1 445 coarse steps before the
+# averaging window of the judging runs.
+RAMPA_CONJUNTO = tuple(37 * k for k in range(16))
 
 
 def calibrar(fech: Fechamento, seeds=TREINO_SEEDS[:3], fatores=(1.0, 10.0, 100.0, 1000.0),
