@@ -322,6 +322,18 @@ def calibrar_esteira(fech: Fechamento, seeds=TREINO_SEEDS[:3],
 RAMPA_CONJUNTO = tuple(37 * k for k in range(16))
 
 
+def _conferir_conjunto(passos_grossos: int, conjunto: int) -> None:
+    """An ensemble's longest ramp (1000 coarse steps plus its offset) must end
+    before the averaging window of the coarse runs (from passos_grossos // 3)."""
+    if not 1 <= conjunto <= len(RAMPA_CONJUNTO):
+        raise ValueError(f"conjunto must be 1..{len(RAMPA_CONJUNTO)}, not {conjunto}")
+    fim = 1000 + RAMPA_CONJUNTO[conjunto - 1]
+    if fim >= passos_grossos // 3:
+        raise ValueError(f"the ramps of a {conjunto}-member ensemble end at coarse step {fim}, "
+                         f"inside the averaging window from step {passos_grossos // 3}; "
+                         f"run at least {3 * (fim + 1)} coarse steps")
+
+
 def calibrar(fech: Fechamento, seeds=TREINO_SEEDS[:3], fatores=(1.0, 10.0, 100.0, 1000.0),
              passos_grossos: int = 9000, log=print, conjunto: int = 1) -> dict:
     """Solver in the loop: scale the trained output by each factor, run the
@@ -332,7 +344,7 @@ def calibrar(fech: Fechamento, seeds=TREINO_SEEDS[:3], fatores=(1.0, 10.0, 100.0
     which only a coarse run can measure.  conjunto > 1: each coarse Cd is the
     mean of that many realizations (ramp offsets RAMPA_CONJUNTO), since one
     realization's windowed Cd scatters by several per cent."""
-    assert 1 <= conjunto <= len(RAMPA_CONJUNTO)
+    _conferir_conjunto(passos_grossos, conjunto)
     finos, grossos = {}, {}
     for sd in seeds:
         grossos[sd] = montar(sd, 1)
@@ -372,7 +384,7 @@ def avaliar(fech: Fechamento, seed: int = AVALIACAO_SEED, passos_grossos: int = 
     offsets RAMPA_CONJUNTO), and the summary adds the ensemble-mean Cd error
     with its standard error: the windowed Cd of one realization of these
     unsteady coarse wakes scatters by several per cent."""
-    assert 1 <= conjunto <= len(RAMPA_CONJUNTO)
+    _conferir_conjunto(passos_grossos, conjunto)
     fino, grosso = montar(seed, 2), montar(seed, 1)
     out = {}
     desde = passos_grossos // 3

@@ -58,3 +58,15 @@ def test_model_files_keep_the_activation(tmp_path):
     assert velho.ativacao == "silu" and velho.c_struct().mlp.activation == 0
     with pytest.raises(ValueError):
         Fechamento(f.params, f.shift, f.scale, f.lo, f.hi, 1e-3, "tanh").c_struct()
+
+
+def test_ensemble_ramps_end_before_the_averaging_window():
+    """Every ensemble size fits the judging runs (9 000 coarse steps, window
+    from step 3 000); a run too short for its ensemble is refused."""
+    import pytest
+    for k in range(1, len(fc.RAMPA_CONJUNTO) + 1):
+        fc._conferir_conjunto(9000, k)
+    with pytest.raises(ValueError, match="averaging window"):
+        fc._conferir_conjunto(4000, 16)
+    with pytest.raises(ValueError):
+        fc._conferir_conjunto(9000, len(fc.RAMPA_CONJUNTO) + 1)
