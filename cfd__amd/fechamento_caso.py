@@ -323,15 +323,15 @@ RAMPA_CONJUNTO = tuple(37 * k for k in range(16))
 
 
 def _conferir_conjunto(passos_grossos: int, conjunto: int) -> None:
-    """An ensemble's longest ramp (1000 coarse steps plus its offset) must end
-    before the averaging window of the coarse runs (from passos_grossos // 3)."""
+    """An ensemble's longest ramp (1000 coarse steps plus its offset) must be
+    over when the averaging window of the coarse runs opens (passos_grossos // 3)."""
     if not 1 <= conjunto <= len(RAMPA_CONJUNTO):
         raise ValueError(f"conjunto must be 1..{len(RAMPA_CONJUNTO)}, not {conjunto}")
     fim = 1000 + RAMPA_CONJUNTO[conjunto - 1]
-    if fim >= passos_grossos // 3:
+    if fim > passos_grossos // 3:
         raise ValueError(f"the ramps of a {conjunto}-member ensemble end at coarse step {fim}, "
                          f"inside the averaging window from step {passos_grossos // 3}; "
-                         f"run at least {3 * (fim + 1)} coarse steps")
+                         f"run at least {3 * fim} coarse steps")
 
 
 def calibrar(fech: Fechamento, seeds=TREINO_SEEDS[:3], fatores=(1.0, 10.0, 100.0, 1000.0),

@@ -66,6 +66,7 @@ def test_ensemble_ramps_end_before_the_averaging_window():
     import pytest
     for k in range(1, len(fc.RAMPA_CONJUNTO) + 1):
         fc._conferir_conjunto(9000, k)
+    fc._conferir_conjunto(3000, 1)          # the profile run's length, one realization
     with pytest.raises(ValueError, match="averaging window"):
         fc._conferir_conjunto(4000, 16)
     with pytest.raises(ValueError):
