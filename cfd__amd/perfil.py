@@ -105,6 +105,11 @@ def resumo(a) -> None:
     per_cell = {"k_dilate": 2, "k_seed": 2, "k_flood_tile": 3, "k_skin": 3, "k_final": 3,
                 "k_prep_types": 4 * 19 * 2 + 1, "k_wd_rows": 2, "k_wd_cols": 3,
                 "k_wd_planes": 3}
+    # the macro kernels: 19 populations read, one float4 written per cell
+    # (the type byte excluded, as for the step); k_pull_macros first, since
+    # k_macros is a part of its name
+    bpp = 4 if j["precisao"] == "fp32" else 2
+    per_cell = {"k_pull_macros": 19 * bpp + 16, "k_macros": 19 * bpp + 16, **per_cell}
     lines = [f"# Kernels around the step, {N // (160 * 128)}x160x128 vehicle "
              f"({N / 1e6:.2f} M cells, {j['triangles']} triangles, {j['precisao']})", "",
              "Commands (run from the repository root on one MI355X):", "", "```",
@@ -155,7 +160,7 @@ def resumo(a) -> None:
         lines += ["", "## HBM traffic (--pmc, per dispatch)", "",
                   f"- copy-kernel calibration (1 GiB each way): FETCH factor {f_fac:.4f}, "
                   f"WRITE factor {w_fac:.4f}", ""]
-        for short in ("k_forces", "k_mass", "k_flood_tile", "k_raster"):
+        for short in ("k_forces", "k_mass", "k_pull_macros", "k_flood_tile", "k_raster"):
             kf = next((k for k in fe if short in k), None)
             kw = next((k for k in wr if short in k), None)
             if kf is None or kw is None:
@@ -235,6 +240,16 @@ def main():
         for _ in range(a.repeticoes):
             s.calcularDistanciaParede()
         t_parede = (time.perf_counter() - t0) / a.repeticoes
+        # the macro field: read after each step (k_pull_macros: that step's
+        # pull and moments over the set it read), and atualizarMacros
+        # (k_macros: the moments of the current set); kernel times come from
+        # the trace
+        for _ in range(a.repeticoes):
+            s.passo(1)
+            s.lerMacros()
+        for _ in range(a.repeticoes):
+            s.atualizarMacros()
+        s.sincronizar()
     finally:
         s.destruir()
     bpp = 4 if a.precisao == "fp32" else 2
